@@ -1,0 +1,170 @@
+"""Benchmark: whole-volume reconstruction of the phi+z flying-focal-spot scan
+(736 ch x 32->64 rows, 10 rotations, pitch 0.8, 512^2 x 224 raw slices in 7
+blocks -> 40 slices of 3 mm in HU) with the scan resident in HBM.
+
+One step = one complete reconstruction: per block adaptive filter, rebin,
+ramp filter and backprojection, then reorder + thickening + HU.
+Prints one JSON line (see keys below).  --form mixed selects the precision
+form (BF16 hi+lo MFMA filter, BF16 filtered projections, FP32 accumulation).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BP_BYTES_PER_UPDATE = 16.0   # 4 FP32 bilinear taps per voxel-update
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c4"])
+    ap.add_argument("--form", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-columns", type=int, default=1024, help="voxel columns in the CPU baseline sample")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_traffic_c1.json"))
+    return ap.parse_args()
+
+
+def cpu_baseline(mr, raw_block, taps, n_cols):
+    """FP32 CPU restatement (oracle/liboracle32.so, one thread) on one block:
+    rebin + FFT ramp filter of the whole block and the backprojection of
+    n_cols sampled voxel columns (all 32 slices), extrapolated to the block."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import ctbangbang_amd as cb
+    import oracle
+    s = cb.block_sizes(mr)
+    t0 = time.perf_counter()
+    reb = oracle.rebin(mr, raw_block, double=False)
+    reb = oracle.filter_fft(reb, taps).astype(np.float32)
+    t1 = time.perf_counter()
+    rng = np.random.default_rng(0)
+    idx = rng.choice(mr.rp.nx * mr.rp.ny, size=n_cols, replace=False)
+    cx, cy = (idx // mr.rp.ny).astype(np.int32), (idx % mr.rp.ny).astype(np.int32)
+    oracle.backproject(mr, reb, oracle.BP_GPU, cols=(cx, cy), double=False)
+    t2 = time.perf_counter()
+    t_block = (t1 - t0) + (t2 - t1) * (mr.rp.nx * mr.rp.ny) / n_cols
+    return {
+        "value": s.voxel_updates_block / t_block,
+        "unit": "voxel-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"one block of the workload: FP32 rebin + FFT ramp filter of the full block "
+                   f"({t1 - t0:.1f} s) and backprojection of {n_cols} of {mr.rp.nx * mr.rp.ny} voxel columns "
+                   f"x 32 slices ({t2 - t1:.1f} s), extrapolated to the block"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(blocks shard with no exchange across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import ctbangbang_amd as cb
+    from ctbangbang_amd import scans
+
+    tok = getattr(scans, args.config + "_tokens")()
+    d = tempfile.mkdtemp()
+    prm = os.path.join(d, "bench.prm")
+    cb.write_prm(prm, tok)
+    mr = cb.metadata_from_prm(prm)
+    ctx = cb.Context(0)
+    mixed = args.form == "mixed"
+    ctx.set_options(filter_mode=cb.FILTER_BF16X3 if mixed else cb.FILTER_F32,
+                    proj_dtype=cb.PROJ_BF16 if mixed else cb.PROJ_F32,
+                    mu_water=float(tok.get("MuWater", 0.0)),
+                    adaptive_filter=1 if float(tok["AdaptiveFiltration"]) > 0 else 0)
+    n_read = int(tok["Readings"])
+    frame = mr.cg.n_rows_raw * mr.cg.n_channels
+    raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda:0")
+    ctx.synth_scan_dev(mr, scans.phantom_for(tok, {"c0": 0, "c1": 1, "c4": 4}[args.config]), 0, n_read, raw)
+    s = cb.block_sizes(mr)
+    final = torch.empty(s.final_stack, dtype=torch.float32, device="cuda:0")
+    image = torch.empty(s.image_stack, dtype=torch.float32, device="cuda:0")
+    vu_blocks = []
+    for b in range(mr.ri.n_blocks):
+        cb.plan_block(mr, b)
+        vu_blocks.append(cb.block_sizes(mr).voxel_updates_block)
+    vu_total = float(sum(vu_blocks))
+
+    for _ in range(args.warmup):
+        ctx.reconstruct_dev(mr, raw, n_read, final, image)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.reconstruct_dev(mr, raw, n_read, final, image)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+
+    # per-kernel device time of the last step (HIP events on the ctx stream)
+    fam = {}
+    for f in ("adaptive", "rebin", "filter", "backproject", "finalize"):
+        ms, n = ctx.kernel_time(f)
+        fam[f] = {"ms_total": round(ms, 3), "launches": n}
+    bp_ms, bp_n = ctx.kernel_time("backproject")
+    bp_avg_s = bp_ms / max(bp_n, 1) / 1e3
+    vu_launch = vu_blocks[-1]
+    achieved = BP_BYTES_PER_UPDATE * vu_launch / bp_avg_s / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        try:
+            traffic = json.load(open(args.traffic_file)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    line = {
+        "metric": "voxel-updates/s (end-to-end, whole volume, scan resident in HBM)",
+        "value": vu_total / dt,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "s_per_volume": dt,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16-storage/f32" if mixed else "f32",
+        "data": "synthetic (analytic Shepp-Logan-type phantom + 8 seeded ellipsoids, Poisson noise I0=1e5)",
+        "config": {"workload": f"{args.config}: {int(mr.rp.nx)}^2 x {int(mr.ri.n_slices_recon)} raw slices, "
+                               f"{int(mr.ri.n_blocks)} blocks, n_ffs={int(mr.ri.n_ffs)}, "
+                               f"{int(mr.cg.n_rows)} rows, {n_read} readings",
+                   "form": args.form, "voxel_updates": vu_total, "parallelism": "1 GPU"},
+        "backproject_updates_per_s": vu_launch / bp_avg_s,
+        "kernels_last_step": fam,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "bp_kernel (backprojection)",
+                     "per_unit": "16 B per voxel-update (4 FP32 bilinear taps)",
+                     "avg_launch_ms": bp_avg_s * 1e3},
+    }
+    if not args.no_cpu_baseline:
+        cb.plan_block(mr, 0)
+        s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+        rb = raw[s0 * frame:(s0 + n) * frame].cpu().numpy()
+        line["cpu_baseline"] = cpu_baseline(mr, rb, cb.load_filter(mr), args.cpu_columns)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+// Adaptive raw-data filtering (Kachelriess-Kalender style), reference
+// src/preprocessing.cu:31-156 and include/preprocessing.cuh:32-285.
+//
+// The reference runs one thread per view for every step (a serial loop over
+// the 23 552 samples of a view in filter_projections).  Here each step is
+// parallelised to its natural width while keeping the reference's float
+// arithmetic in the same order (including the fused multiply-adds nvcc forms
+// for "acc += a*b"), so results are bit-identical to a sequential FP32
+// restatement:
+//   1. view maximum         one workgroup per view, tree max (order-free)
+//   2. 18-degree smoothing  one thread per view, sequential fmaf sum
+//   3. 180-degree min/max + eccentricity   one thread per view
+//   4. threshold bisection  one wavefront per view, exact integer counts
+//   5. 5x5x5 triangle smoothing of samples above threshold: one thread per
+//      sample (coalesced along channels)
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+// Only the explicit fmaf calls may fuse: every other multiply-add rounds
+// twice, as in the sequential FP32 statement of the algorithm.
+#pragma clang fp contract(off)
+
+namespace ctbb {
+
+namespace {
+
+__global__ __launch_bounds__(256) void af_sup(const float *raw, float *sup, size_t frame) {
+    __shared__ float red[256];
+    const size_t p = blockIdx.x;
+    const float *f = raw + p * frame;
+    float m = 0.0f;   // reference starts from 0.0f (preprocessing.cuh:75)
+    for (size_t i = threadIdx.x; i < frame; i += 256) m = fmaxf(m, f[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sup[p] = red[0];
+}
+
+// reference preprocessing.cuh:87-113
+__global__ void af_smooth(const float *sup, float *sm, int n_proj, int n_avg) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_proj) return;
+    const int off = n_avg / 2;
+    const int max_idx = n_proj - 1;
+    const float inv = 1.0f / static_cast<float>(n_avg);
+    float acc = 0.0f;
+    for (int i = 0; i < n_avg; i++) {
+        int idx = p - off + i;
+        idx = idx < 0 ? 0 : (idx > max_idx ? max_idx : idx);
+        acc = fmaf(inv, sup[idx], acc);
+    }
+    sm[p] = acc;
+}
+
+// reference preprocessing.cuh:115-163
+__global__ void af_ecc(const float *sm, float *mx, float *mn, float *ecc, int n_proj, int n_range) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_proj) return;
+    const int off = n_range / 2;
+    const int max_idx = n_proj - 1;
+    float hi = 0.0f, lo = 10000.0f;
+    for (int i = 0; i < n_range; i++) {
+        int idx = p - off + i;
+        idx = idx < 0 ? 0 : (idx > max_idx ? max_idx : idx);
+        const float v = sm[idx];
+        if (v > hi) hi = v;
+        if (v < lo) lo = v;
+    }
+    mx[p] = hi;
+    mn[p] = lo;
+    float e = 1 - lo / hi;
+    e = (e - 0.3f) / (0.5f - 0.3f);
+    ecc[p] = e > 1 ? 1.0f : (e < 0 ? 0.0f : e);
+}
+
+// rhs of reference preprocessing.cuh:32-62 evaluated by one wavefront
+__device__ float af_rhs_wave(const float *array, int start_idx, int max_idx, int numel_range, float T,
+                             float inv_pi_npf_a, float inv_npf) {
+    int diff = start_idx + (numel_range - 1);
+    if (diff < numel_range - 1) {
+        numel_range += start_idx;
+        start_idx = 0;
+    }
+    if (diff > max_idx) numel_range -= (diff - max_idx);
+    int cnt = 0;
+    for (int i = threadIdx.x & 63; i < numel_range; i += 64) cnt += array[start_idx + i] > T ? 1 : 0;
+    for (int s = 32; s > 0; s >>= 1) cnt += __shfl_xor(cnt, s, 64);
+    return inv_pi_npf_a * static_cast<float>(cnt) * inv_npf;
+}
+
+// reference preprocessing.cuh:167-206; one wavefront per view
+__global__ __launch_bounds__(256) void af_thresh(const float *sm, const float *mx, const float *mn,
+                                                 const float *ecc, float *thr, int n_proj, int n_range,
+                                                 int n_proj_ffs, float strength) {
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= n_proj) return;
+    const int off = n_range / 2;
+    const int max_idx = n_proj - 1;
+    const float inv_pi = 1.0f / 3.1415368979f;
+    const float inv_npf = 1.0f / static_cast<float>(n_proj_ffs);
+    float lower = mn[p], upper = mx[p];
+    float T = (mx[p] + mn[p]) / 2.0f;
+    const float lhs = 0.03f * strength * ecc[p];
+    float rhs = af_rhs_wave(sm, p - off, max_idx, n_range, T, inv_pi, inv_npf);
+    int iter = 0;
+    while (static_cast<double>(fabsf(lhs - rhs) / lhs) > 0.01) {
+        if (iter >= 20) break;
+        iter += 1;
+        if (rhs > lhs) lower = T;
+        else if (rhs < lhs) upper = T;
+        else break;
+        T = (lower + upper) / 2.0f;
+        rhs = af_rhs_wave(sm, p - off, max_idx, n_range, T, inv_pi, inv_npf);
+    }
+    if ((threadIdx.x & 63) == 0) thr[p] = T;
+}
+
+// reference preprocessing.cuh:208-285 (3D branch, __FILTER_3D__ defined)
+__global__ __launch_bounds__(256) void af_apply(const float *raw, const float *thr, float *out, int n_proj,
+                                                int n_rows_raw, int nch) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // channel
+    const int j = blockIdx.y;                                // row
+    const int p = blockIdx.z;                                // view
+    if (i >= nch) return;
+    const size_t ps = static_cast<size_t>(nch) * n_rows_raw;
+    const size_t idx = ps * p + static_cast<size_t>(j) * nch + i;
+    const float T = thr[p];
+    const float x = raw[idx];
+    if (!(x > T)) {
+        out[idx] = x;
+        return;
+    }
+    float norm = 0.0f, val = 0.0f;
+    for (int ii = 0; ii < 5; ii++) {
+        const int fc = i + ii - 2;
+        for (int jj = 0; jj < 5; jj++) {
+            const int fr = j + jj - 2;
+            for (int kk = 0; kk < 5; kk++) {
+                const int fp = p + kk - 2;
+                if (fr < 0 || fc < 0 || fp < 0 || fr >= n_rows_raw || fc >= nch || fp >= n_proj) continue;
+                // triangle_3d(5, ii, jj, kk), ceil(5/2) = 3: peak at offset +1
+                const float t = static_cast<float>((3 - abs(ii - 3)) + (3 - abs(jj - 3)) + (3 - abs(kk - 3)) + 1);
+                norm += t;
+                val = fmaf(raw[ps * fp + static_cast<size_t>(nch) * fr + fc], t, val);
+            }
+        }
+    }
+    out[idx] = (1.0f / norm) * val;
+}
+
+int odd_window(float frac, int n_proj_ffs) {
+    int n = static_cast<int>(ceilf(frac * static_cast<float>(n_proj_ffs)));
+    return n + (1 - n % 2);
+}
+
+}  // namespace
+
+int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
+    if (!a.raw || !a.out || !a.work || a.raw == a.out) return CTBB_ERR_ARG;
+    if (a.n_proj <= 0 || a.n_rows_raw <= 0 || a.nch <= 0 || a.n_proj_ffs <= 0) return CTBB_ERR_SHAPE;
+    if (a.n_proj > 65535 || a.n_rows_raw > 65535) return CTBB_ERR_SHAPE;  // grid.z / grid.y limits
+    const size_t np = static_cast<size_t>(a.n_proj);
+    float *sup = a.work, *sm = sup + np, *mx = sm + np, *mn = mx + np, *ecc = mn + np, *thr = ecc + np;
+    const size_t frame = static_cast<size_t>(a.n_rows_raw) * a.nch;
+    const int n_avg = odd_window(18.0f / 360.0f, a.n_proj_ffs);
+    const int n_range = odd_window(180.0f / 360.0f, a.n_proj_ffs);
+    const unsigned g1 = static_cast<unsigned>((a.n_proj + 255) / 256);
+    hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
+    hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
+    hipLaunchKernelGGL(af_ecc, dim3(g1), dim3(256), 0, stream, sm, mx, mn, ecc, a.n_proj, n_range);
+    hipLaunchKernelGGL(af_thresh, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
+                       n_range, a.n_proj_ffs, a.strength);
+    hipLaunchKernelGGL(af_apply, dim3((a.nch + 255) / 256, a.n_rows_raw, a.n_proj), dim3(256), 0, stream, a.raw,
+                       thr, a.out, a.n_proj, a.n_rows_raw, a.nch);
+    return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
+}
+
+}  // namespace ctbb

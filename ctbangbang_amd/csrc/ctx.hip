@@ -1,0 +1,677 @@
+// Execution context and the C-ABI of the four GPU stages.
+//
+// The context owns one non-blocking HIP stream (default priority), grow-only
+// device workspaces, cached coordinate maps / filter taps, and HIP events
+// recorded around every launch so callers can read per-kernel device time.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ctbb_hip.h"
+#include "geometry.h"
+#include "kernels.h"
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n) {
+        if (n <= bytes) return CTBB_OK;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, n) != hipSuccess) return CTBB_ERR_NOMEM;
+        bytes = n;
+        return CTBB_OK;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ctbb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ctbb_options opt{};
+    DevBuf raw_in, raw_af, rebin_tmp, rebin_out, image_block, image_stack, af_work;
+    DevBuf vtab, maps, taps, fin_tab;
+    // cached host-side keys
+    std::string taps_key, maps_key;
+    int map_kind = 0, map_sets = 0;
+    std::map<std::string, std::vector<EventPair>> events;
+    std::vector<EventPair> free_events;
+
+    EventPair begin(const char *family) {
+        EventPair e;
+        if (!free_events.empty()) {
+            e = free_events.back();
+            free_events.pop_back();
+        } else {
+            hipEventCreate(&e.a);
+            hipEventCreate(&e.b);
+        }
+        hipEventRecord(e.a, stream);
+        events[family].push_back(e);
+        return e;
+    }
+    void end(const EventPair &e) { hipEventRecord(e.b, stream); }
+    void reset_events() {
+        for (auto &kv : events)
+            for (auto &e : kv.second) free_events.push_back(e);
+        events.clear();
+    }
+};
+
+namespace {
+
+using namespace ctbb;
+
+int hip_ok(hipError_t e) { return e == hipSuccess ? CTBB_OK : CTBB_ERR_HIP; }
+
+int upload(ctbb_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
+    int st = dst.ensure(bytes);
+    if (st) return st;
+    if (hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return CTBB_ERR_HIP;
+    // the host staging buffer may be reused as soon as this returns
+    return hip_ok(hipStreamSynchronize(ctx->stream));
+}
+
+int check_geometry(const recon_metadata *mr) {
+    if (!mr) return CTBB_ERR_ARG;
+    const ct_geom &cg = mr->cg;
+    if (mr->ri.n_ffs == 0 || cg.n_channels == 0 || cg.n_rows == 0 || cg.n_proj_turn < 2) return CTBB_ERR_ARG;
+    if (cg.n_channels_oversampled != 2 * cg.n_channels) return CTBB_ERR_SHAPE;
+    if (mr->ri.n_proj_pull % mr->ri.n_ffs) return CTBB_ERR_SHAPE;
+    if (mr->ri.n_proj_pull / mr->ri.n_ffs <= 2 * cg.add_projections) return CTBB_ERR_SHAPE;
+    return CTBB_OK;
+}
+
+// Filter taps of the prm's ReconKernel, cached per (install dir, channels, kernel).
+int ensure_taps(ctbb_ctx *ctx, const recon_metadata *mr) {
+    std::string key = std::string(mr->install_dir) + "|" + std::to_string(mr->cg.n_channels) + "|" +
+                      std::to_string(mr->rp.recon_kernel);
+    if (key == ctx->taps_key && ctx->taps.p) return CTBB_OK;
+    std::vector<float> t(2 * mr->cg.n_channels_oversampled);
+    int st = ctbb_load_filter(mr, t.data());
+    if (st) return st;
+    st = upload(ctx, ctx->taps, t.data(), t.size() * sizeof(float));
+    if (st) return st;
+    ctx->taps_key = key;
+    return CTBB_OK;
+}
+
+int ensure_maps(ctbb_ctx *ctx, const recon_metadata *mr) {
+    const ct_geom &cg = mr->cg;
+    char key[512];
+    std::snprintf(key, sizeof(key), "%zu|%zu|%zu|%a|%a|%a|%a|%a|%a|%zu|%d|%d|%d|%a", cg.n_channels, cg.n_rows,
+                  cg.n_proj_ffs, cg.r_f, cg.src_to_det, cg.fan_angle_increment, cg.central_channel, cg.anode_angle,
+                  mr->rp.coll_slicewidth, mr->ri.n_ffs, mr->rp.z_ffs, mr->rp.phi_ffs, cg.reverse_row_interleave,
+                  static_cast<double>(cg.n_proj_turn));
+    if (ctx->maps_key == key && ctx->maps.p) return CTBB_OK;
+    std::vector<float> bidx, aoff;
+    int kind = 0, sets = 0;
+    int st = rebin_maps(*mr, bidx, aoff, &kind, &sets);
+    if (st) return st;
+    std::vector<float> both(bidx.size() + aoff.size());
+    std::memcpy(both.data(), bidx.data(), bidx.size() * sizeof(float));
+    std::memcpy(both.data() + bidx.size(), aoff.data(), aoff.size() * sizeof(float));
+    st = upload(ctx, ctx->maps, both.data(), both.size() * sizeof(float));
+    if (st) return st;
+    ctx->maps_key = key;
+    ctx->map_kind = kind;
+    ctx->map_sets = sets;
+    return CTBB_OK;
+}
+
+int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out) {
+    int st = ensure_maps(ctx, mr);
+    if (st) return st;
+    const ct_geom &cg = mr->cg;
+    RebinArgs a{};
+    a.raw = d_raw;
+    a.out = d_out;
+    const size_t nos = cg.n_channels_oversampled;
+    a.bidx = ctx->maps.as<float>();
+    a.aoff = ctx->maps.as<float>() + static_cast<size_t>(ctx->map_sets) * nos;
+    a.kind = ctx->map_kind;
+    a.n_ffs = static_cast<int>(mr->ri.n_ffs);
+    a.nch = static_cast<int>(cg.n_channels);
+    a.nos = static_cast<int>(nos);
+    a.n_rows_raw = static_cast<int>(cg.n_rows_raw);
+    a.n_rows = static_cast<int>(cg.n_rows);
+    a.n_proj = static_cast<int>(mr->ri.n_proj_pull / mr->ri.n_ffs);
+    a.add = static_cast<int>(cg.add_projections);
+    a.n_proj_out = a.n_proj - 2 * a.add;
+    a.reverse_rows = cg.reverse_row_interleave;
+    EventPair e = ctx->begin("rebin");
+    st = launch_rebin(a, ctx->stream);
+    ctx->end(e);
+    return st;
+}
+
+int do_filter(ctbb_ctx *ctx, const float *d_taps, int n, size_t rows, const float *d_in, void *d_out, int out_bf16) {
+    FilterArgs f{};
+    f.in = d_in;
+    f.out = d_out;
+    f.taps = d_taps;
+    f.rows = rows;
+    f.n = n;
+    f.mode = ctx->opt.filter_mode;
+    f.out_bf16 = out_bf16;
+    EventPair e = ctx->begin("filter");
+    int st = launch_filter(f, ctx->stream);
+    ctx->end(e);
+    return st;
+}
+
+int fill_bp_args(const recon_metadata *mr, BpArgs &a) {
+    const ct_geom &cg = mr->cg;
+    const recon_params &rp = mr->rp;
+    if (rp.nx % 16 || rp.ny % 16 || rp.nx == 0 || rp.ny == 0) return CTBB_ERR_SHAPE;
+    if (mr->ri.n_slices_block != 32) return CTBB_ERR_SHAPE;
+    a.nx = static_cast<int>(rp.nx);
+    a.ny = static_cast<int>(rp.ny);
+    a.nos = static_cast<int>(cg.n_channels_oversampled);
+    a.n_rows = static_cast<int>(cg.n_rows);
+    a.half = static_cast<int>(cg.n_proj_turn / 2);
+    a.n_ht = bp_n_half_turns(*mr);
+    if (a.n_ht <= 0) return CTBB_ERR_SHAPE;
+    a.tiles_y = a.ny / 16;
+    a.n_tiles = (a.nx / 16) * a.tiles_y;
+    // reference include/backproject.cuh:59-64
+    const double dx = static_cast<double>(rp.recon_fov) / rp.nx, dy = static_cast<double>(rp.recon_fov) / rp.ny;
+    a.xs = static_cast<float>(cg.table_direction == -1 ? dx : -dx);
+    a.xc = (rp.nx - 1) / 2.0f;
+    a.x0 = rp.x_origin;
+    a.ys = static_cast<float>(dy);
+    a.yc = (rp.ny - 1) / 2.0f;
+    a.y0 = rp.y_origin;
+    const double ds = static_cast<double>(cg.r_f) * cg.fan_angle_increment / 2.0;
+    a.inv_ds = static_cast<float>(1.0 / ds);
+    a.two_cc = 2.0f * cg.central_channel;
+    a.inv_rf = static_cast<float>(1.0 / cg.r_f);
+    a.rf2 = cg.r_f * cg.r_f;
+    a.k_asin = static_cast<float>(cg.z_rot / (2.0 * static_cast<double>(kPiBp)));
+    a.coll = rp.coll_slicewidth;
+    a.tanc = static_cast<float>(std::tan(static_cast<double>(cg.theta_cone) / 2.0));
+    a.neg_td = static_cast<float>(-cg.table_direction);
+    a.hr = (static_cast<float>(cg.n_rows) - 1.0f) / 2.0f;
+    a.row_max = static_cast<float>(cg.n_rows) - 1.0001f;
+    const double dmax = dx > dy ? dx : dy;
+    a.half_w = static_cast<float>(7.5 * std::sqrt(2.0) * dmax / ds + 1.0);
+    int wl = 2;
+    while ((1 << wl) < 2.0 * a.half_w + 4.0) wl++;
+    a.wch_log2 = wl;
+    a.acq_half = rp.acq_fov / 2.0f;
+    a.out_scale = static_cast<float>(2.0 * static_cast<double>(kPiBp) / cg.n_proj_turn);
+    return CTBB_OK;
+}
+
+int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin, const BpView *d_vtab,
+                   float *d_out) {
+    BpArgs a{};
+    int st = fill_bp_args(mr, a);
+    if (st) return st;
+    if (ctx->opt.proj_dtype == CTBB_PROJ_BF16) a.rebin_bf16 = static_cast<const uint16_t *>(d_rebin);
+    else a.rebin = static_cast<const float *>(d_rebin);
+    a.vtab = d_vtab;
+    a.out = d_out;
+    EventPair e = ctx->begin("backproject");
+    st = launch_backproject(a, ctx->stream);
+    ctx->end(e);
+    return st;
+}
+
+// Final-slice tap lists: reference src/finalize_image_stack.cu:45-105 with the
+// block reversal of :56-63 folded into the source slice index.
+int finalize_table(const recon_metadata *mr, std::vector<int> &src, std::vector<float> &w, int *max_taps,
+                   int *n_final) {
+    const recon_params &rp = mr->rp;
+    const recon_info &ri = mr->ri;
+    const int nb = static_cast<int>(ri.n_slices_block);
+    const int n_raw = nb * static_cast<int>(ri.n_blocks);
+    if (n_raw <= 0 || !mr->table_positions || rp.n_readings <= 1000) return CTBB_ERR_ARG;
+    float d = rp.end_pos - rp.start_pos;
+    int recon_direction = d == 0.0f ? 1 : static_cast<int>(std::fabs(d) / d);
+    if (recon_direction != 1 && recon_direction != -1) recon_direction = 1;
+    double tdd = mr->table_positions[1000] - mr->table_positions[0];
+    int table_direction = tdd == 0.0 ? 1 : static_cast<int>(std::fabs(tdd) / tdd);
+    const int nf = ctbb_n_slices_final(mr);
+    std::vector<float> raw_loc(n_raw), wt(n_raw);
+    for (int i = 0; i < n_raw; i++) raw_loc[i] = ri.recon_start_pos + recon_direction * i * rp.coll_slicewidth;
+    std::vector<std::vector<std::pair<int, float>>> lists(nf);
+    int mt = 1;
+    for (int k = 0; k < nf; k++) {
+        const float loc = rp.start_pos + recon_direction * k * rp.slice_thickness;
+        float sum = 0.0f;
+        for (int i = 0; i < n_raw; i++) {
+            wt[i] = std::fmax(0.0f, 1.0f - std::fabs(raw_loc[i] - loc) / rp.slice_thickness);
+            sum += wt[i];
+        }
+        for (int i = 0; i < n_raw; i++) {
+            if (wt[i] == 0.0f) continue;
+            const int b = i / nb, z = i % nb;
+            const int s = recon_direction != table_direction ? b * nb + (nb - 1 - z) : i;
+            lists[k].push_back({s, wt[i] / sum});
+        }
+        if (static_cast<int>(lists[k].size()) > mt) mt = static_cast<int>(lists[k].size());
+    }
+    src.assign(static_cast<size_t>(nf) * mt, -1);
+    w.assign(static_cast<size_t>(nf) * mt, 0.0f);
+    for (int k = 0; k < nf; k++)
+        for (size_t t = 0; t < lists[k].size(); t++) {
+            src[static_cast<size_t>(k) * mt + t] = lists[k][t].first;
+            w[static_cast<size_t>(k) * mt + t] = lists[k][t].second;
+        }
+    *max_taps = mt;
+    *n_final = nf;
+    return CTBB_OK;
+}
+
+int do_finalize(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_image, float *d_final) {
+    std::vector<int> src;
+    std::vector<float> w;
+    int mt = 0, nf = 0;
+    int st = finalize_table(mr, src, w, &mt, &nf);
+    if (st) return st;
+    std::vector<char> blob(src.size() * sizeof(int) + w.size() * sizeof(float));
+    std::memcpy(blob.data(), src.data(), src.size() * sizeof(int));
+    std::memcpy(blob.data() + src.size() * sizeof(int), w.data(), w.size() * sizeof(float));
+    st = upload(ctx, ctx->fin_tab, blob.data(), blob.size());
+    if (st) return st;
+    FinalizeArgs f{};
+    f.image = d_image;
+    f.out = d_final;
+    f.src_slice = ctx->fin_tab.as<int>();
+    f.weight = reinterpret_cast<const float *>(ctx->fin_tab.as<char>() + src.size() * sizeof(int));
+    f.max_taps = mt;
+    f.n_final = nf;
+    f.nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
+    f.mu_water = ctx->opt.mu_water;
+    EventPair e = ctx->begin("finalize");
+    st = launch_finalize(f, ctx->stream);
+    ctx->end(e);
+    return st;
+}
+
+int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out) {
+    const ct_geom &cg = mr->cg;
+    AdaptiveArgs a{};
+    a.raw = d_raw;
+    a.out = d_out;
+    a.n_proj = static_cast<int>(mr->ri.n_proj_pull);
+    a.n_rows_raw = static_cast<int>(cg.n_rows_raw);
+    a.nch = static_cast<int>(cg.n_channels);
+    a.n_proj_ffs = static_cast<int>(cg.n_proj_ffs);
+    a.strength = mr->rp.adaptive_filtration_s;
+    int st = ctx->af_work.ensure(6 * static_cast<size_t>(a.n_proj) * sizeof(float));
+    if (st) return st;
+    a.work = ctx->af_work.as<float>();
+    EventPair e = ctx->begin("adaptive");
+    st = launch_adaptive(a, ctx->stream);
+    ctx->end(e);
+    return st;
+}
+
+size_t proj_elem_bytes(const ctbb_ctx *ctx) { return ctx->opt.proj_dtype == CTBB_PROJ_BF16 ? 2 : 4; }
+
+int rebin_filter_dev(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, void *d_rebin) {
+    int st = check_geometry(mr);
+    if (st) return st;
+    ctbb_sizes sz;
+    ctbb_block_sizes(mr, &sz);
+    st = ensure_taps(ctx, mr);
+    if (st) return st;
+    st = ctx->rebin_tmp.ensure(sz.rebin * sizeof(float));
+    if (st) return st;
+    st = do_rebin(ctx, mr, d_raw, ctx->rebin_tmp.as<float>());
+    if (st) return st;
+    const int n = static_cast<int>(mr->cg.n_channels_oversampled);
+    return do_filter(ctx, ctx->taps.as<float>(), n, sz.n_proj_out * mr->cg.n_rows, ctx->rebin_tmp.as<float>(),
+                     d_rebin, ctx->opt.proj_dtype == CTBB_PROJ_BF16);
+}
+
+int backproject_dev(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin, float *d_img) {
+    int st = check_geometry(mr);
+    if (st) return st;
+    if (!mr->tube_angles) return CTBB_ERR_ARG;
+    std::vector<BpView> tab;
+    bp_view_table(*mr, tab);
+    st = upload(ctx, ctx->vtab, tab.data(), tab.size() * sizeof(BpView));
+    if (st) return st;
+    return do_backproject(ctx, mr, d_rebin, ctx->vtab.as<BpView>(), d_img);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctbb_ctx_create(int device, ctbb_ctx **out) {
+    if (!out) return CTBB_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return CTBB_ERR_HIP;
+    if (device >= 0) {
+        if (device >= n) return CTBB_ERR_ARG;
+        if (hipSetDevice(device) != hipSuccess) return CTBB_ERR_HIP;
+    }
+    ctbb_ctx *c = new ctbb_ctx();
+    hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CTBB_ERR_HIP;
+    }
+    c->opt.filter_mode = CTBB_FILTER_F32;
+    c->opt.proj_dtype = CTBB_PROJ_F32;
+    c->opt.mu_water = 0.0f;
+    c->opt.bp_tile = 0;
+    c->opt.bp_views_per_pass = 0;
+    c->opt.adaptive_filter = 1;
+    *out = c;
+    return CTBB_OK;
+}
+
+void ctbb_ctx_destroy(ctbb_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->rebin_out, &c->image_block, &c->image_stack,
+                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->fin_tab})
+        b->release();
+    c->reset_events();
+    for (auto &e : c->free_events) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ctbb_ctx_get_options(const ctbb_ctx *c, ctbb_options *o) {
+    if (!c || !o) return CTBB_ERR_ARG;
+    *o = c->opt;
+    return CTBB_OK;
+}
+
+int ctbb_ctx_set_options(ctbb_ctx *c, const ctbb_options *o) {
+    if (!c || !o) return CTBB_ERR_ARG;
+    if (o->filter_mode < CTBB_FILTER_F32 || o->filter_mode > CTBB_FILTER_F16X3) return CTBB_ERR_ARG;
+    if (o->proj_dtype != CTBB_PROJ_F32 && o->proj_dtype != CTBB_PROJ_BF16) return CTBB_ERR_ARG;
+    c->opt = *o;
+    return CTBB_OK;
+}
+
+void *ctbb_ctx_stream(ctbb_ctx *c) { return c ? static_cast<void *>(c->stream) : nullptr; }
+
+int ctbb_ctx_synchronize(ctbb_ctx *c) {
+    if (!c) return CTBB_ERR_ARG;
+    return hip_ok(hipStreamSynchronize(c->stream));
+}
+
+int ctbb_ctx_kernel_time(ctbb_ctx *c, const char *family, float *total_ms, int *launches) {
+    if (!c || !family || !total_ms || !launches) return CTBB_ERR_ARG;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return CTBB_ERR_HIP;
+    *total_ms = 0.0f;
+    *launches = 0;
+    auto it = c->events.find(family);
+    if (it == c->events.end()) return CTBB_OK;
+    for (auto &e : it->second) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e.a, e.b) != hipSuccess) return CTBB_ERR_HIP;
+        *total_ms += ms;
+        *launches += 1;
+    }
+    return CTBB_OK;
+}
+
+int ctbb_adaptive_filter_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_raw, float *d_out) {
+    if (!c || !mr || !d_raw || !d_out) return CTBB_ERR_ARG;
+    int st = check_geometry(mr);
+    if (st) return st;
+    hipSetDevice(c->device);
+    c->reset_events();
+    return do_adaptive(c, mr, d_raw, d_out);
+}
+
+int ctbb_rebin_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_raw, float *d_rebin) {
+    if (!c || !mr || !d_raw || !d_rebin) return CTBB_ERR_ARG;
+    int st = check_geometry(mr);
+    if (st) return st;
+    hipSetDevice(c->device);
+    c->reset_events();
+    return do_rebin(c, mr, d_raw, d_rebin);
+}
+
+int ctbb_filter_rows_dev(ctbb_ctx *c, const float *d_taps, int n, size_t n_rows_total, const float *d_in,
+                         void *d_out) {
+    if (!c || !d_taps || !d_in || !d_out || n <= 0) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    return do_filter(c, d_taps, n, n_rows_total, d_in, d_out, c->opt.proj_dtype == CTBB_PROJ_BF16);
+}
+
+int ctbb_rebin_filter_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_raw, void *d_rebin) {
+    if (!c || !mr || !d_raw || !d_rebin) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    return rebin_filter_dev(c, mr, d_raw, d_rebin);
+}
+
+int ctbb_backproject_dev(ctbb_ctx *c, const struct recon_metadata *mr, const void *d_rebin, float *d_image_block) {
+    if (!c || !mr || !d_rebin || !d_image_block) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    return backproject_dev(c, mr, d_rebin, d_image_block);
+}
+
+int ctbb_finalize_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_image, float *d_final) {
+    if (!c || !mr || !d_image || !d_final) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    return do_finalize(c, mr, d_image, d_final);
+}
+
+int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_raw, size_t n_raw_readings,
+                         float *d_image, float *d_final) {
+    if (!c || !mr || !d_raw || !d_final || !mr->table_positions || !mr->tube_angles) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    const ct_geom &cg = mr->cg;
+    const int n_blocks = static_cast<int>(mr->ri.n_blocks);
+    if (n_blocks <= 0) return CTBB_ERR_ARG;
+    const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
+    const size_t frame = cg.n_rows_raw * cg.n_channels;
+    int st;
+    if (!d_image) {
+        st = c->image_stack.ensure(mr->ri.n_slices_recon * nxy * sizeof(float));
+        if (st) return st;
+        d_image = c->image_stack.as<float>();
+    }
+    // Per-block view tables, computed up front and uploaded once.
+    std::vector<BpView> all;
+    std::vector<size_t> off(n_blocks);
+    for (int b = 0; b < n_blocks; b++) {
+        ctbb::plan_block(mr, b);
+        st = check_geometry(mr);
+        if (st) return st;
+        std::vector<BpView> t;
+        bp_view_table(*mr, t);
+        off[b] = all.size();
+        all.insert(all.end(), t.begin(), t.end());
+    }
+    st = upload(c, c->vtab, all.data(), all.size() * sizeof(BpView));
+    if (st) return st;
+    st = ensure_taps(c, mr);
+    if (st) return st;
+    st = ensure_maps(c, mr);
+    if (st) return st;
+    for (int b = 0; b < n_blocks; b++) {
+        ctbb::plan_block(mr, b);
+        ctbb_sizes sz;
+        ctbb_block_sizes(mr, &sz);
+        const long start = static_cast<long>(static_cast<int>(mr->ri.idx_pull_start));
+        const long n = static_cast<long>(mr->ri.n_proj_pull);
+        const float *win;
+        if (start >= 0 && start + n <= static_cast<long>(n_raw_readings)) {
+            win = d_raw + static_cast<size_t>(start) * frame;
+        } else {
+            // pull reaches outside the scan: zero-padded copy
+            st = c->raw_in.ensure(static_cast<size_t>(n) * frame * sizeof(float));
+            if (st) return st;
+            float *dst = c->raw_in.as<float>();
+            if (hipMemsetAsync(dst, 0, static_cast<size_t>(n) * frame * sizeof(float), c->stream) != hipSuccess)
+                return CTBB_ERR_HIP;
+            long lo = start < 0 ? 0 : start, hi = start + n;
+            if (hi > static_cast<long>(n_raw_readings)) hi = static_cast<long>(n_raw_readings);
+            if (hi > lo &&
+                hipMemcpyAsync(dst + static_cast<size_t>(lo - start) * frame, d_raw + static_cast<size_t>(lo) * frame,
+                               static_cast<size_t>(hi - lo) * frame * sizeof(float), hipMemcpyDeviceToDevice,
+                               c->stream) != hipSuccess)
+                return CTBB_ERR_HIP;
+            win = dst;
+        }
+        if (c->opt.adaptive_filter) {
+            st = c->raw_af.ensure(static_cast<size_t>(n) * frame * sizeof(float));
+            if (st) return st;
+            st = do_adaptive(c, mr, win, c->raw_af.as<float>());
+            if (st) return st;
+            win = c->raw_af.as<float>();
+        }
+        st = c->rebin_tmp.ensure(sz.rebin * sizeof(float));
+        if (st) return st;
+        st = do_rebin(c, mr, win, c->rebin_tmp.as<float>());
+        if (st) return st;
+        st = c->rebin_out.ensure(sz.rebin * proj_elem_bytes(c));
+        if (st) return st;
+        st = do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), sz.n_proj_out * cg.n_rows,
+                       c->rebin_tmp.as<float>(), c->rebin_out.p, c->opt.proj_dtype == CTBB_PROJ_BF16);
+        if (st) return st;
+        st = do_backproject(c, mr, c->rebin_out.p, c->vtab.as<BpView>() + off[b],
+                            d_image + static_cast<size_t>(b) * mr->ri.n_slices_block * nxy);
+        if (st) return st;
+    }
+    return do_finalize(c, mr, d_image, d_final);
+}
+
+// --- reference host-memory contract ---------------------------------------
+
+int ctbb_adaptive_filter(ctbb_ctx *c, struct recon_metadata *mr) {
+    if (!c || !mr || !mr->ctd.raw) return CTBB_ERR_ARG;
+    int st = check_geometry(mr);
+    if (st) return st;
+    hipSetDevice(c->device);
+    c->reset_events();
+    const size_t n = mr->ri.n_proj_pull * mr->cg.n_rows_raw * mr->cg.n_channels;
+    if ((st = c->raw_in.ensure(n * sizeof(float))) || (st = c->raw_af.ensure(n * sizeof(float)))) return st;
+    if (hipMemcpyAsync(c->raw_in.p, mr->ctd.raw, n * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    st = do_adaptive(c, mr, c->raw_in.as<float>(), c->raw_af.as<float>());
+    if (st) return st;
+    if (hipMemcpyAsync(mr->ctd.raw, c->raw_af.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    return hip_ok(hipStreamSynchronize(c->stream));
+}
+
+int ctbb_rebin_filter(ctbb_ctx *c, struct recon_metadata *mr) {
+    if (!c || !mr || !mr->ctd.raw) return CTBB_ERR_ARG;
+    int st = check_geometry(mr);
+    if (st) return st;
+    hipSetDevice(c->device);
+    c->reset_events();
+    const size_t n = mr->ri.n_proj_pull * mr->cg.n_rows_raw * mr->cg.n_channels;
+    if ((st = c->raw_in.ensure(n * sizeof(float)))) return st;
+    if (hipMemcpyAsync(c->raw_in.p, mr->ctd.raw, n * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    ctbb_sizes sz;
+    ctbb_block_sizes(mr, &sz);
+    void *d = nullptr;
+    if (hipMalloc(&d, sz.rebin * proj_elem_bytes(c)) != hipSuccess) return CTBB_ERR_NOMEM;
+    st = rebin_filter_dev(c, mr, c->raw_in.as<float>(), d);
+    if (st) {
+        hipFree(d);
+        return st;
+    }
+    if (mr->flags.testing && mr->ctd.rebin && c->opt.proj_dtype == CTBB_PROJ_F32) {
+        if (hipMemcpyAsync(mr->ctd.rebin, d, sz.rebin * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return CTBB_ERR_HIP;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        hipFree(d);
+        return CTBB_ERR_HIP;
+    }
+    mr->ctd.d_rebin = static_cast<float *>(d);
+    return CTBB_OK;
+}
+
+int ctbb_backproject(ctbb_ctx *c, struct recon_metadata *mr) {
+    if (!c || !mr || !mr->ctd.d_rebin || !mr->ctd.image || mr->ri.cb.block_idx < 1) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    const size_t nblk = mr->ri.n_slices_block * static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
+    int st = c->image_block.ensure(nblk * sizeof(float));
+    if (st) return st;
+    st = backproject_dev(c, mr, mr->ctd.d_rebin, c->image_block.as<float>());
+    if (st) return st;
+    float *dst = mr->ctd.image + static_cast<size_t>(mr->ri.cb.block_idx - 1) * nblk;
+    if (hipMemcpyAsync(dst, c->image_block.p, nblk * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    st = hip_ok(hipStreamSynchronize(c->stream));
+    hipFree(mr->ctd.d_rebin);
+    mr->ctd.d_rebin = nullptr;
+    return st;
+}
+
+int ctbb_finalize_image_stack(ctbb_ctx *c, struct recon_metadata *mr) {
+    if (!c || !mr || !mr->ctd.image) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
+    const size_t n_img = mr->ri.n_slices_recon * nxy;
+    const int nf = ctbb_n_slices_final(mr);
+    int st = c->image_stack.ensure(n_img * sizeof(float));
+    if (st) return st;
+    DevBuf fin;
+    if ((st = fin.ensure(static_cast<size_t>(nf) * nxy * sizeof(float)))) return st;
+    if (hipMemcpyAsync(c->image_stack.p, mr->ctd.image, n_img * sizeof(float), hipMemcpyHostToDevice, c->stream) !=
+        hipSuccess) {
+        fin.release();
+        return CTBB_ERR_HIP;
+    }
+    st = do_finalize(c, mr, c->image_stack.as<float>(), fin.as<float>());
+    if (st == CTBB_OK) {
+        std::free(mr->ctd.final_image_stack);
+        mr->ctd.final_image_stack = static_cast<float *>(std::calloc(static_cast<size_t>(nf) * nxy, sizeof(float)));
+        if (!mr->ctd.final_image_stack) st = CTBB_ERR_NOMEM;
+        else if (hipMemcpyAsync(mr->ctd.final_image_stack, fin.p, static_cast<size_t>(nf) * nxy * sizeof(float),
+                                hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            st = CTBB_ERR_HIP;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && st == CTBB_OK) st = CTBB_ERR_HIP;
+    fin.release();
+    return st;
+}
+
+int ctbb_synth_scan_dev(ctbb_ctx *c, const struct recon_metadata *mr, const ctbb_phantom_params *pp, long first,
+                        long count, float *d_out) {
+    if (!c || !mr || !pp || !d_out) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    return launch_synth(*mr, *pp, first, count, d_out, c->stream);
+}
+
+}  // extern "C"

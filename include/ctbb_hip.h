@@ -1,0 +1,264 @@
+/*
+ * ctbb_hip.h — C-ABI of the MI355X reconstruction hot path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns a ctbb_status (0 = success) instead of calling exit() the way the
+ * reference program does.  The drop-in wrappers with the reference's own
+ * C++ names (ctbb_compat.h) map these codes back to the reference's exits.
+ *
+ * Three groups of entry points:
+ *   1. planning / I/O on `struct recon_metadata` (host only, no GPU needed);
+ *   2. the four hot-path stages with the reference's host-memory contract
+ *      (raw in host memory, d_rebin on the device between rebin and
+ *      backprojection, image and final stack in host memory);
+ *   3. the same stages on device-resident buffers, for callers that keep the
+ *      scan in HBM (whole-volume pipelines, benchmarks, batching).
+ *
+ * Data layouts (float32 unless noted):
+ *   raw pull        [n_proj_pull][n_rows_raw][n_channels]        channel fastest
+ *   rebinned block  [n_proj_out][n_rows][n_channels_oversampled] (n_proj_out =
+ *                   n_proj_pull/n_ffs - 2*add_projections), ramp-filtered
+ *   image block     [32][nx][ny]  (slice, xi, yi; yi fastest)
+ *   image stack     [n_slices_recon][nx][ny]
+ *   final stack     [n_slices_final][nx][ny]
+ */
+#ifndef CTBB_HIP_H
+#define CTBB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "recon_structs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CTBB_OK = 0,
+    CTBB_ERR_ARG = 1,       /* invalid argument / null pointer */
+    CTBB_ERR_IO = 2,        /* file missing, unreadable or unwritable */
+    CTBB_ERR_PRM = 3,       /* an essential prm value is missing or zero */
+    CTBB_ERR_RANGE = 4,     /* requested slices outside the allowed data range */
+    CTBB_ERR_SCANNER = 5,   /* scanner cannot be parsed */
+    CTBB_ERR_FILTER = 6,    /* filter file missing or unreadable */
+    CTBB_ERR_SHAPE = 7,     /* geometry outside what the kernels support */
+    CTBB_ERR_HIP = 8,       /* HIP runtime error (no device, launch failure) */
+    CTBB_ERR_NOMEM = 9,     /* host or device allocation failed */
+    CTBB_ERR_UNSUPPORTED = 10 /* raw file type without a reader in this build */
+} ctbb_status;
+
+const char *ctbb_strerror(int status);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Planning and I/O (host only).                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Parse a prm file, convert TableDir to table_dir and check the essential
+ * fields.  Replaces configure_recon_params (reference src/setup.cu:108-176);
+ * CTBB_ERR_PRM where the reference exits with code 1, CTBB_ERR_IO where the
+ * prm is missing (reference main.cu:165-168 exits 13). */
+int ctbb_configure_recon_params(const char *prm_file, struct recon_params *out);
+
+/* Linear attenuation of water (1/mm) given by the "MuWater:" token of a prm
+ * file, or 0 when absent.  The token is unknown to the reference parser. */
+int ctbb_prm_mu_water(const char *prm_file, float *mu_water);
+
+/* Resolve cwd / install_dir / output path and check the raw file is readable
+ * (1) and the output writable (2): reference src/setup.cu:40-106.
+ * install_dir is the directory holding resources/ (the package directory of
+ * this library unless the environment variable CTBB_INSTALL_DIR is set). */
+int ctbb_configure_paths(struct recon_metadata *mr);
+
+/* Scanner geometry from a scanner file (path, or file name under
+ * install_dir/resources/scanners) or a hard-coded scanner 0/1/2:
+ * reference src/setup.cu:178-383. */
+int ctbb_configure_ct_geom(struct recon_metadata *mr, struct ct_geom *out);
+
+/* Tube angles, table positions, slice/block plan, first pull and the host
+ * arrays raw/rebin/image: reference src/setup.cu:385-603.  Reads only the raw
+ * file's existence for binary (FileType 0) data. */
+int ctbb_configure_reconstruction(struct recon_metadata *mr);
+
+/* Same as ctbb_configure_reconstruction but never allocates ctd.raw,
+ * ctd.rebin or ctd.image and does not open the raw file (tube angles and
+ * table positions are analytic for FileType 0).  For device-resident use. */
+int ctbb_plan_reconstruction(struct recon_metadata *mr);
+
+/* Pull range of the next block; post-increments ri.cb.block_idx and
+ * reallocates ctd.raw / ctd.rebin: reference src/setup.cu:605-679. */
+int ctbb_update_block_info(struct recon_metadata *mr);
+
+/* Same arithmetic as ctbb_update_block_info for block `block_idx` (0-based),
+ * without touching any allocation; ri.cb.block_idx becomes block_idx+1. */
+int ctbb_plan_block(struct recon_metadata *mr, int block_idx);
+
+/* Read the current pull into ctd.raw (FileType 0 binary only; reference
+ * src/setup.cu:681-767).  Frames past the end of the file read as zeros. */
+int ctbb_extract_projections(struct recon_metadata *mr);
+
+/* Write final_image_stack to OutputDir/OutputFile and free every host
+ * allocation of mr: reference src/setup.cu:769-787. */
+int ctbb_finish_and_cleanup(struct recon_metadata *mr);
+
+/* reference src/setup.cu:806-825 */
+int ctbb_array_search(float key, const double *array, int numel_array, int search_type);
+
+/* Number of final (thickened) slices: reference src/finalize_image_stack.cu:76 */
+int ctbb_n_slices_final(const struct recon_metadata *mr);
+
+/* Load the 2*n_channels_oversampled ramp-filter taps for rp.recon_kernel from
+ * install_dir/resources/filters (reference src/rebin_filter.cu:922-959). */
+int ctbb_load_filter(const struct recon_metadata *mr, float *taps);
+
+/* Closed-form apodised ramp filter (reference resources/ramp.m:1-16):
+ * taps[j] = ramp(j - n, ds, c, a) for j in [0, 2n), n = n_channels_oversampled. */
+int ctbb_ramp_filter(int n, double ds, double c, double a, float *taps);
+
+/* ------------------------------------------------------------------------ */
+/* 2./3. GPU execution.                                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ctbb_ctx ctbb_ctx;
+
+typedef enum {
+    CTBB_FILTER_F32 = 0,        /* FP32 Toeplitz product, FP32 accumulation */
+    CTBB_FILTER_BF16X3 = 1,     /* BF16 hi+lo split operands on MFMA, FP32 accumulation */
+    CTBB_FILTER_F16X3 = 2       /* FP16 hi+lo split operands on MFMA, FP32 accumulation */
+} ctbb_filter_mode;
+
+typedef enum {
+    CTBB_PROJ_F32 = 0,          /* filtered projections stored as float32 */
+    CTBB_PROJ_BF16 = 1          /* filtered projections stored as bfloat16 */
+} ctbb_proj_dtype;
+
+typedef struct ctbb_options_s {
+    int filter_mode;            /* ctbb_filter_mode */
+    int proj_dtype;             /* ctbb_proj_dtype */
+    float mu_water;             /* >0: convert the final stack to HU */
+    int bp_tile;                /* backprojection tile edge (0 = tuned default) */
+    int bp_views_per_pass;      /* angles staged per LDS pass (0 = default) */
+    int adaptive_filter;        /* 1 = run the adaptive raw-data filter (default) */
+} ctbb_options;
+
+/* Create a context on HIP device `device` (-1: current device).  The context
+ * owns a non-blocking stream, device workspaces and per-device tables. */
+int ctbb_ctx_create(int device, ctbb_ctx **out);
+void ctbb_ctx_destroy(ctbb_ctx *ctx);
+int ctbb_ctx_get_options(const ctbb_ctx *ctx, ctbb_options *opt);
+int ctbb_ctx_set_options(ctbb_ctx *ctx, const ctbb_options *opt);
+/* The HIP stream (hipStream_t) the context launches on, as an opaque pointer. */
+void *ctbb_ctx_stream(ctbb_ctx *ctx);
+/* Block until all work queued on the context's stream has finished. */
+int ctbb_ctx_synchronize(ctbb_ctx *ctx);
+/* Device time of a kernel family during the most recent top-level call on
+ * the context, from HIP events recorded on the context stream around each
+ * launch.  family: "adaptive", "rebin", "filter", "backproject", "finalize".
+ * Synchronises the stream; *total_ms = summed duration, *launches = count. */
+int ctbb_ctx_kernel_time(ctbb_ctx *ctx, const char *family, float *total_ms, int *launches);
+
+/* --- stages with the reference's host-memory contract -------------------- */
+
+/* Adaptive raw-data filter, in place on mr->ctd.raw.
+ * Replaces adaptive_filter_kk (reference include/preprocessing.h:26). */
+int ctbb_adaptive_filter(ctbb_ctx *ctx, struct recon_metadata *mr);
+
+/* Rebin (n/p/z/a flying focal spot) + ramp filter of mr->ctd.raw; allocates
+ * mr->ctd.d_rebin on the device.  Replaces rebin_filter
+ * (reference include/rebin_filter.h:27). */
+int ctbb_rebin_filter(ctbb_ctx *ctx, struct recon_metadata *mr);
+
+/* Weighted-FBP backprojection of mr->ctd.d_rebin into block
+ * (ri.cb.block_idx-1) of mr->ctd.image; frees d_rebin.  Replaces backproject
+ * (reference include/backproject.h:26). */
+int ctbb_backproject(ctbb_ctx *ctx, struct recon_metadata *mr);
+
+/* Block reorder + slice thickening (+ HU when options.mu_water > 0) of
+ * mr->ctd.image into a newly allocated mr->ctd.final_image_stack.  Replaces
+ * finalize_image_stack (reference include/finalize_image_stack.h:26). */
+int ctbb_finalize_image_stack(ctbb_ctx *ctx, struct recon_metadata *mr);
+
+/* --- device-resident stages ---------------------------------------------- */
+
+/* Element counts of the current block's buffers (from mr->ri / mr->cg / mr->rp). */
+typedef struct {
+    size_t raw_pull;        /* n_proj_pull * n_rows_raw * n_channels */
+    size_t rebin;           /* n_proj_out * n_rows * n_channels_oversampled */
+    size_t image_block;     /* 32 * nx * ny */
+    size_t image_stack;     /* n_slices_recon * nx * ny */
+    size_t final_stack;     /* n_slices_final * nx * ny */
+    size_t n_proj_out;
+    int n_half_turns;
+    int n_slices_final;
+    double voxel_updates_block; /* nx*ny*32*(n_proj_turn/2)*n_half_turns */
+} ctbb_sizes;
+
+int ctbb_block_sizes(const struct recon_metadata *mr, ctbb_sizes *out);
+
+/* d_out[pull] = adaptive filter of d_raw[pull] (out of place, both device). */
+int ctbb_adaptive_filter_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
+                             const float *d_raw, float *d_out);
+
+/* d_rebin = ramp-filtered rebinned block from the device raw pull.  d_rebin is
+ * float* for CTBB_PROJ_F32 and uint16_t* (bfloat16) for CTBB_PROJ_BF16. */
+int ctbb_rebin_filter_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
+                          const float *d_raw, void *d_rebin);
+
+/* Rebin only (no ramp filter), float32 output.  Exposes the rebinned
+ * sinogram the reference writes as its rebin test dump before filtering. */
+int ctbb_rebin_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
+                   const float *d_raw, float *d_rebin);
+
+/* Ramp filter rows of length n_channels_oversampled in place:
+ * rows[r][l] <- sum_k taps[n + l - k] * rows[r][k]. d_out may alias d_in only
+ * for CTBB_PROJ_F32 output. */
+int ctbb_filter_rows_dev(ctbb_ctx *ctx, const float *d_taps, int n, size_t n_rows_total,
+                         const float *d_in, void *d_out);
+
+/* Backproject a filtered block into d_image_block [32][nx][ny] (overwritten). */
+int ctbb_backproject_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
+                         const void *d_rebin, float *d_image_block);
+
+/* Reorder + thicken (+HU) d_image [n_slices_recon][nx][ny] into d_final. */
+int ctbb_finalize_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
+                      const float *d_image, float *d_final);
+
+/* Whole reconstruction from a device-resident scan: every block of the plan
+ * in mr (ctbb_plan_reconstruction), then finalize, into d_final.  d_raw holds
+ * readings [0, n_raw_readings) of the scan; pulls outside that range read
+ * zeros.  d_image (n_slices_recon*nx*ny floats) may be NULL (context
+ * workspace is used).  Leaves mr->ri.cb at the last block. */
+int ctbb_reconstruct_dev(ctbb_ctx *ctx, struct recon_metadata *mr,
+                         const float *d_raw, size_t n_raw_readings,
+                         float *d_image, float *d_final);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic helical scans (test data generator).                           */
+/* ------------------------------------------------------------------------ */
+
+/* Analytic line integrals of a 3D Shepp-Logan-type head phantom plus
+ * `n_random` seeded ellipsoids for every reading in [first, first+count) of
+ * the scan planned in mr (geometry, FFS focal positions, tube angles, table
+ * positions), with Poisson noise at i0 incident photons (i0 <= 0: no noise).
+ * Output layout [count][n_rows_raw][n_channels].  `_dev` writes device memory,
+ * the host variant host memory; both evaluate the same FP64 expressions (they
+ * can differ in the last bits where the host and device math libraries do). */
+typedef struct {
+    uint64_t phantom_seed;
+    uint64_t noise_seed;
+    int n_random;           /* extra random ellipsoids (8 in the standard scans) */
+    double i0;              /* incident photons per sample; <=0 disables noise */
+    double scale_mm;        /* phantom unit length in mm (head ~ 0.69 x 0.92 units) */
+    double z_center;        /* phantom centre in table coordinates (mm) */
+    double mu_water;        /* attenuation of the phantom's "brain" level (1/mm) */
+} ctbb_phantom_params;
+
+int ctbb_synth_scan(const struct recon_metadata *mr, const ctbb_phantom_params *pp,
+                    long first, long count, float *out);
+int ctbb_synth_scan_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
+                        const ctbb_phantom_params *pp, long first, long count, float *d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
