@@ -43,14 +43,10 @@ __device__ __forceinline__ float taper(float t) {
     return p;
 }
 
-// Stierstorfer weight W(q), Q = 0.6 (reference include/backproject.cuh:33-46):
-// exactly 1 for |q| < 0.6, exactly 0 for |q| >= 1.
-__device__ __forceinline__ float stier_w(float q) {
-    float aq = fabsf(q);
-    float t = fminf(fmaxf((aq - 0.6f) * 2.5f, 0.0f), 1.0f);
-    float w = taper(t);
-    w = aq < 0.6f ? 1.0f : w;
-    return aq < 1.0f ? w : 0.0f;
+// Taper part of the Stierstorfer weight W(q), Q = 0.6 (reference
+// include/backproject.cuh:33-46), for 0.6 <= |q| < 1.
+__device__ __forceinline__ float stier_taper(float aq) {
+    return taper(__builtin_amdgcn_fmed3f((aq - 0.6f) * 2.5f, 0.0f, 1.0f));
 }
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
@@ -82,17 +78,14 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const float xm = a.xs * (static_cast<float>(tx * 16) + 7.5f - a.xc) + a.x0;
     const float ym = a.ys * (static_cast<float>(ty * 16) + 7.5f - a.yc) + a.y0;
 
-    float s[NZ];
+    float s[NZ], st[NZ], ht[NZ];
 #pragma unroll
-    for (int z = 0; z < NZ; z++) s[z] = 0.0f;
+    for (int z = 0; z < NZ; z++) { s[z] = 0.0f; st[z] = 0.0f; ht[z] = 0.0f; }
 
     const int n_stage = a.n_rows << a.wch_log2;
+    const unsigned stride_b = static_cast<unsigned>(stride) * 4u;
 
     for (int i = 0; i < a.half; i++) {
-        float st[NZ], ht[NZ];
-#pragma unroll
-        for (int z = 0; z < NZ; z++) { st[z] = 0.0f; ht[z] = 0.0f; }
-
         for (int k = 0; k < a.n_ht; k++) {
             const int v = i + k * a.half;
             const BpView vw = a.vtab[v];
@@ -100,55 +93,77 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
             const float pm = (xm * vw.sin_t - ym * vw.cos_t) * a.inv_ds + a.two_cc;
             const int p_lo = static_cast<int>(floorf(pm - a.half_w)) - 1;
 
-            __syncthreads();  // previous view fully consumed
+            // per-column geometry, shared by the NZ slices:
+            // q(z) = A z + B, q = qhat of reference include/backproject.cuh:84
+            const float phat = x * vw.sin_t - y * vw.cos_t;
+            const float pl = phat * a.inv_ds + a.two_cc - static_cast<float>(p_lo);
+            int c0 = static_cast<int>(floorf(pl));
+            const float fp = pl - static_cast<float>(c0);
+            c0 = min(max(c0, 0), wch - 2);
+            const float asn = asinf(phat * a.inv_rf);
+            const float dz0 = vw.c_v + a.k_asin * asn + static_cast<float>(z0) * a.coll;
+            const float lhat = sqrtf(a.rf2 - phat * phat) - x * vw.cos_t - y * vw.sin_t;
+            const float g = a.neg_td * __builtin_amdgcn_rcpf(lhat * a.tanc);
+            const float A = g * a.coll;
+            const float B = g * dz0;
+            // W(q) = 0 for |q| >= 1: slices outside (-1, 1) add exactly nothing
+            const float qe = fmaf(A, static_cast<float>(NZ - 1), B);
+            const bool vis = fminf(B, qe) < 1.0f && fmaxf(B, qe) > -1.0f;
+
+            // also the barrier that retires the previous view's LDS reads
+            if (!__syncthreads_or(vis)) continue;
             const size_t vbase = static_cast<size_t>(v) * a.n_rows * a.nos;
             for (int e = tid; e < n_stage; e += 256) {
                 const int r = e >> a.wch_log2;
                 const int c = e & (wch - 1);
-                const int g = p_lo + c;
+                const int gch = p_lo + c;
                 float val = 0.0f;
-                if (g >= 0 && g < a.nos) {
-                    const size_t gi = vbase + static_cast<size_t>(r) * a.nos + g;
+                if (gch >= 0 && gch < a.nos) {
+                    const size_t gi = vbase + static_cast<size_t>(r) * a.nos + gch;
                     val = kBf16 ? bf16_to_f32(a.rebin_bf16[gi]) : a.rebin[gi];
                 }
                 lds[r * stride + c] = val;
             }
             __syncthreads();
 
-            // per-column geometry, shared by the NZ slices
-            const float phat = x * vw.sin_t - y * vw.cos_t;
-            const float pl = phat * a.inv_ds + a.two_cc - static_cast<float>(p_lo);
-            int c0 = static_cast<int>(floorf(pl));
-            float fp = pl - static_cast<float>(c0);
-            c0 = min(max(c0, 0), wch - 2);
-            const float asn = asinf(phat * a.inv_rf);
-            const float dz0 = vw.c_v + a.k_asin * asn + static_cast<float>(z0) * a.coll;
-            const float lhat = sqrtf(a.rf2 - phat * phat) - x * vw.cos_t - y * vw.sin_t;
-            const float g = a.neg_td / (lhat * a.tanc);
-            const float A = g * a.coll;
-            const float B = g * dz0;
-            const float *col = lds + c0;
-
+            const char *colb = reinterpret_cast<const char *>(lds) + 4u * static_cast<unsigned>(c0);
 #pragma unroll
             for (int z = 0; z < NZ; z++) {
                 const float q = fmaf(A, static_cast<float>(z), B);
-                const float w = stier_w(q);
-                const float rowf = fminf(fmaxf(fmaf(q, a.hr, a.hr), 0.0f), a.row_max);
-                const int r0 = static_cast<int>(rowf);
-                const float fr = rowf - static_cast<float>(r0);
-                const float *p0 = col + r0 * stride;
-                const float v00 = p0[0], v01 = p0[1];
-                const float v10 = p0[stride], v11 = p0[stride + 1];
-                const float top = fmaf(fp, v01 - v00, v00);
-                const float bot = fmaf(fp, v11 - v10, v10);
-                const float val = fmaf(fr, bot - top, top);
-                st[z] = fmaf(val, w, st[z]);
-                ht[z] += w;
+                const float aq = fabsf(q);
+                // q varies by < 1% across a wavefront: the slice is skipped when
+                // no lane of the wavefront sees it, and the taper is evaluated
+                // only when some lane is in 0.6 <= |q| < 1 (wave-uniform votes)
+                const bool zv = vis && aq < 1.0f;
+                if (__any(zv)) {
+                    float w = zv ? 1.0f : 0.0f;
+                    const bool tap = zv && aq >= 0.6f;
+                    if (__any(tap)) {
+                        const float tw = stier_taper(aq);
+                        w = tap ? tw : w;
+                    }
+                    const float rowf = __builtin_amdgcn_fmed3f(fmaf(q, a.hr, a.hr), 0.0f, a.row_max);
+                    const unsigned r0 = static_cast<unsigned>(rowf);
+                    const float fr = __builtin_amdgcn_fractf(rowf);
+                    const float *p0 = reinterpret_cast<const float *>(colb + __umul24(r0, stride_b));
+                    const float *p1 = reinterpret_cast<const float *>(colb + __umul24(r0, stride_b) + stride_b);
+                    const float v00 = p0[0], v01 = p0[1];
+                    const float v10 = p1[0], v11 = p1[1];
+                    const float top = fmaf(fp, v01 - v00, v00);
+                    const float bot = fmaf(fp, v11 - v10, v10);
+                    const float val = fmaf(fr, bot - top, top);
+                    st[z] = fmaf(val, w, st[z]);
+                    ht[z] += w;
+                }
             }
         }
+        // s += (1/h_t) s_t where h_t != 0 (reference include/backproject.cuh:104-108)
 #pragma unroll
-        for (int z = 0; z < NZ; z++)
-            if (ht[z] != 0.0f) s[z] += (1.0f / ht[z]) * st[z];
+        for (int z = 0; z < NZ; z++) {
+            s[z] += ht[z] != 0.0f ? __builtin_amdgcn_rcpf(ht[z]) * st[z] : 0.0f;
+            st[z] = 0.0f;
+            ht[z] = 0.0f;
+        }
     }
 
     const bool in_fov = sqrtf(x * x + y * y) < a.acq_half;
@@ -167,12 +182,15 @@ int launch_backproject(const BpArgs &a, hipStream_t stream) {
     if (a.n_tiles != (a.nx / 16) * (a.ny / 16) || a.tiles_y != a.ny / 16) return CTBB_ERR_ARG;
     const size_t lds_bytes = static_cast<size_t>(a.n_rows) * ((1 << a.wch_log2) + 1) * sizeof(float);
     if (lds_bytes > 64 * 1024) return CTBB_ERR_SHAPE;
-    constexpr int NZ = 32;
-    dim3 grid(a.n_tiles, 32 / NZ);
-    if (a.rebin_bf16)
-        hipLaunchKernelGGL((bp_kernel<NZ, true>), grid, dim3(256), lds_bytes, stream, a);
-    else
-        hipLaunchKernelGGL((bp_kernel<NZ, false>), grid, dim3(256), lds_bytes, stream, a);
+    if (a.nz != 16 && a.nz != 32) return CTBB_ERR_ARG;
+    dim3 grid(a.n_tiles, 32 / a.nz);
+    if (a.nz == 32) {
+        if (a.rebin_bf16) hipLaunchKernelGGL((bp_kernel<32, true>), grid, dim3(256), lds_bytes, stream, a);
+        else hipLaunchKernelGGL((bp_kernel<32, false>), grid, dim3(256), lds_bytes, stream, a);
+    } else {
+        if (a.rebin_bf16) hipLaunchKernelGGL((bp_kernel<16, true>), grid, dim3(256), lds_bytes, stream, a);
+        else hipLaunchKernelGGL((bp_kernel<16, false>), grid, dim3(256), lds_bytes, stream, a);
+    }
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 

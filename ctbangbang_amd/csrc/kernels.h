@@ -21,6 +21,7 @@ struct BpArgs {
     float *out;                // [32][nx][ny]
     int nx, ny, nos, n_rows, half, n_ht;
     int wch_log2;              // staged channel window = 1 << wch_log2
+    int nz;                    // slices per thread: 32 (one z chunk) or 16 (two)
     int tiles_y;               // ny / 16
     int n_tiles;
     float xs, xc, x0;          // x(xi) = xs * (xi - xc) + x0

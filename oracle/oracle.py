@@ -171,6 +171,18 @@ def adaptive_filter(mr, raw):
     return out, thr
 
 
+def adaptive_apply(mr, raw, thresholds):
+    """The 5x5x5 smoothing step alone, with given per-view thresholds."""
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    thr = np.ascontiguousarray(thresholds, dtype=np.float32)
+    out = np.empty_like(raw)
+    l = lib(False)
+    l.orc_adaptive_apply.argtypes = [C.c_void_p] * 4
+    l.orc_adaptive_apply.restype = None
+    l.orc_adaptive_apply(_mr(mr), _addr(raw), _addr(thr), _addr(out))
+    return out
+
+
 def rebin_filter(mr, raw, taps, double=True):
     """Rebin + FFT ramp filter (FP64 filter), float32 result."""
     r = rebin(mr, raw, double)

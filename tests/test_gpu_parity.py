@@ -128,8 +128,9 @@ def test_adaptive_filter_bit_exact(ctx, tmp_path, mini_dir, ffs):
     cb.free_metadata(mr)
 
 
+@pytest.mark.parametrize("nz", [32, 16])
 @pytest.mark.parametrize("table_dir", ["out", "in"])
-def test_backproject_random_sinogram(ctx, tmp_path, mini_dir, table_dir):
+def test_backproject_random_sinogram(ctx, tmp_path, mini_dir, table_dir, nz):
     import oracle
     tok = scans.mini_tokens("none", nrows=8, nx=32, table_dir=table_dir)
     mr = _mr(tmp_path, tok, mini_dir)
@@ -138,13 +139,44 @@ def test_backproject_random_sinogram(ctx, tmp_path, mini_dir, table_dir):
     rng = np.random.default_rng(5)
     reb = rng.standard_normal((s.n_proj_out, mr.cg.n_rows, mr.cg.n_channels_oversampled)).astype(np.float32)
     img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
-    ctx.set_options(proj_dtype=cb.PROJ_F32)
+    ctx.set_options(proj_dtype=cb.PROJ_F32, bp_tile=nz)
     ctx.backproject_dev(mr, dev(reb), img)
     got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
+    ctx.set_options(bp_tile=0)
     want = oracle.backproject(mr, reb, oracle.BP_GPU, double=True)
     err = np.abs(got - want).max() / np.abs(want).max()
-    print(f"bp random ({table_dir}): rel max err {err:.3g}")
+    print(f"bp random ({table_dir}, nz={nz}): rel max err {err:.3g}")
     assert err <= 2e-4
+    cb.free_metadata(mr)
+
+
+@pytest.mark.parametrize("nz", [32, 16])
+def test_backproject_definitionas_block_sampled(ctx, tmp_path, nz):
+    """Full-size phi+z FFS block (512^2, 64 rows): HIP vs FP64 oracle on 400
+    sampled voxel columns of a filtered synthetic scan, in HU."""
+    import oracle
+    tok = scans.c1_tokens()
+    mr = _mr(tmp_path, tok, None)
+    cb.plan_block(mr, 3)
+    s = cb.block_sizes(mr)
+    raw = _raw_pull(ctx, mr, scans.phantom_for(tok, 1))
+    reb = torch.empty(s.rebin, dtype=torch.float32, device="cuda")
+    ctx.set_options(filter_mode=cb.FILTER_F32, proj_dtype=cb.PROJ_F32, bp_tile=nz)
+    ctx.rebin_filter_dev(mr, raw, reb)
+    img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
+    ctx.backproject_dev(mr, reb, img)
+    got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
+    ctx.set_options(bp_tile=0)
+    rng = np.random.default_rng(11)
+    idx = rng.choice(mr.rp.nx * mr.rp.ny, 400, replace=False)
+    cx, cy = idx // mr.rp.ny, idx % mr.rp.ny
+    want = oracle.backproject(mr, host(reb).reshape(s.n_proj_out, mr.cg.n_rows, -1), oracle.BP_GPU,
+                              cols=(cx, cy), double=True)
+    g = got[:, cx, cy].astype(np.float64)
+    w = want[:, cx, cy].astype(np.float64)
+    hu = 1000.0 * (g - w) / 0.02
+    print(f"bp c1 block (nz={nz}): HU max {np.abs(hu).max():.4f} rms {np.sqrt(np.mean(hu ** 2)):.4f}")
+    assert np.abs(hu).max() <= 1.0
     cb.free_metadata(mr)
 
 
