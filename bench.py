@@ -31,7 +31,57 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-columns", type=int, default=1024, help="voxel columns in the CPU baseline sample")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_traffic_c1.json"))
+    ap.add_argument("--stage", default="all", choices=["all", "backproject", "copy"],
+                    help="all: whole volume (the benchmark); backproject: only the backprojection of one block "
+                         "(for counter runs); copy: a 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--bp-tile", type=int, default=32, choices=[16, 32])
     return ap.parse_args()
+
+
+def stage_copy(args):
+    """1 GiB device-to-device copy by an elementwise kernel: the reference
+    point that scales FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"stage": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "ms": dt * 1e3,
+                      "GB_per_s": 8 * n / dt / 1e9}), flush=True)
+
+
+def stage_backproject(args, cb, scans, mr, ctx, raw, frame):
+    """Backprojection of the middle block only, K launches."""
+    import torch
+    cb.plan_block(mr, int(mr.ri.n_blocks) // 2)
+    s = cb.block_sizes(mr)
+    s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+    win = raw[s0 * frame:(s0 + n) * frame]
+    elem = 2 if ctx.options().proj_dtype == cb.PROJ_BF16 else 4
+    reb = torch.empty(s.rebin * elem // 4 + 1, dtype=torch.float32, device="cuda:0")
+    ctx.rebin_filter_dev(mr, win, reb)
+    img = torch.empty(s.image_block, dtype=torch.float32, device="cuda:0")
+    for _ in range(args.warmup):
+        ctx.backproject_dev(mr, reb, img)
+    ctx.synchronize()
+    total, launches = 0.0, 0
+    for _ in range(args.steps):
+        ctx.backproject_dev(mr, reb, img)
+        ms, k = ctx.kernel_time("backproject")
+        total += ms
+        launches += k
+    avg = total / launches / 1e3
+    print(json.dumps({"stage": "backproject", "voxel_updates": s.voxel_updates_block, "avg_launch_ms": avg * 1e3,
+                      "updates_per_s": s.voxel_updates_block / avg,
+                      "gather_GB_per_s": BP_BYTES_PER_UPDATE * s.voxel_updates_block / avg / 1e9,
+                      "bp_tile": args.bp_tile, "form": args.form}), flush=True)
 
 
 def cpu_baseline(mr, raw_block, taps, n_cols):
@@ -72,7 +122,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(blocks shard with no exchange across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    if args.stage == "copy":
+        stage_copy(args)
+        return
     import torch
 
     import ctbangbang_amd as cb
@@ -85,14 +137,19 @@ def main():
     mr = cb.metadata_from_prm(prm)
     ctx = cb.Context(0)
     mixed = args.form == "mixed"
-    ctx.set_options(filter_mode=cb.FILTER_BF16X3 if mixed else cb.FILTER_F32,
+    ctx.set_options(filter_mode=cb.FILTER_F16X3 if mixed else cb.FILTER_F32,
                     proj_dtype=cb.PROJ_BF16 if mixed else cb.PROJ_F32,
                     mu_water=float(tok.get("MuWater", 0.0)),
-                    adaptive_filter=1 if float(tok["AdaptiveFiltration"]) > 0 else 0)
+                    adaptive_filter=1 if float(tok["AdaptiveFiltration"]) > 0 else 0,
+                    bp_tile=args.bp_tile)
     n_read = int(tok["Readings"])
     frame = mr.cg.n_rows_raw * mr.cg.n_channels
     raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda:0")
     ctx.synth_scan_dev(mr, scans.phantom_for(tok, {"c0": 0, "c1": 1, "c4": 4}[args.config]), 0, n_read, raw)
+    if args.stage == "backproject":
+        stage_backproject(args, cb, scans, mr, ctx, raw, frame)
+        ctx.close()
+        return
     s = cb.block_sizes(mr)
     final = torch.empty(s.final_stack, dtype=torch.float32, device="cuda:0")
     image = torch.empty(s.image_stack, dtype=torch.float32, device="cuda:0")

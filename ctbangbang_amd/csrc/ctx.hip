@@ -215,9 +215,6 @@ int fill_bp_args(const recon_metadata *mr, BpArgs &a) {
     a.row_max = static_cast<float>(cg.n_rows) - 1.0001f;
     const double dmax = dx > dy ? dx : dy;
     a.half_w = static_cast<float>(7.5 * std::sqrt(2.0) * dmax / ds + 1.0);
-    int wl = 2;
-    while ((1 << wl) < 2.0 * a.half_w + 4.0) wl++;
-    a.wch_log2 = wl;
     a.acq_half = rp.acq_fov / 2.0f;
     a.out_scale = static_cast<float>(2.0 * static_cast<double>(kPiBp) / cg.n_proj_turn);
     return CTBB_OK;

@@ -40,38 +40,20 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void toeplitz_kernel(FilterArgs a) {
+// FP32 Toeplitz product on v_mfma_f32_16x16x4_f32.
+__global__ __launch_bounds__(256) void toeplitz_f32_kernel(FilterArgs a) {
     extern __shared__ float smem[];
     const int n = a.n;
     float *As = smem;                          // [BM][A_STRIDE]
-    float *Tf = smem + BM * A_STRIDE;          // [2n] taps (F32 mode)
-    uint16_t *Th = reinterpret_cast<uint16_t *>(Tf);         // [2n] hi (split modes)
-    uint16_t *Tl = Th + 2 * n;                                // [2n] lo
+    float *Tf = smem + BM * A_STRIDE;          // [2n] taps
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const size_t m0 = static_cast<size_t>(blockIdx.y) * BM;
     const int n0 = blockIdx.x * BN;
-    const float tscale = MODE == CTBB_FILTER_F16X3 ? 1024.0f : 1.0f;
 
-    // taps -> LDS (split into hi/lo for the split modes)
-    for (int j = tid; j < 2 * n; j += 256) {
-        float t = a.taps[j] * tscale;
-        if (MODE == CTBB_FILTER_F32) {
-            Tf[j] = t;
-        } else if (MODE == CTBB_FILTER_BF16X3) {
-            uint16_t h = f32_to_bf16_rne(t);
-            Th[j] = h;
-            Tl[j] = f32_to_bf16_rne(t - bf16_to_f32(h));
-        } else {
-            _Float16 h = static_cast<_Float16>(t);
-            _Float16 l = static_cast<_Float16>(t - static_cast<float>(h));
-            Th[j] = __builtin_bit_cast(uint16_t, h);
-            Tl[j] = __builtin_bit_cast(uint16_t, l);
-        }
-    }
+    for (int j = tid; j < 2 * n; j += 256) Tf[j] = a.taps[j];
 
     f32x4 acc[2][2];
 #pragma unroll
@@ -80,99 +62,186 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(FilterArgs a) {
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int lr = lane & 15, lk = lane >> 4;
-    const float *X = a.in;
+    const int arow = tid >> 2, aseg = (tid & 3) * 8;
+    const size_t gm = m0 + arow;
+    const bool row_ok = gm < a.rows;
+    const float *xrow = a.in + gm * n + aseg;
+    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
+    if (row_ok) {
+        x0 = reinterpret_cast<const float4 *>(xrow)[0];
+        x1 = reinterpret_cast<const float4 *>(xrow)[1];
+    }
 
     for (int k0 = 0; k0 < n; k0 += BK) {
         __syncthreads();
-        // A tile: 64 rows x 32 floats, 8 per thread (two float4)
         {
-            const int row = tid >> 2, seg = (tid & 3) * 8;
-            const size_t gm = m0 + row;
-            float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-            if (gm < a.rows) {
-                const float4 *src = reinterpret_cast<const float4 *>(X + gm * n + k0 + seg);
-                v0 = src[0];
-                v1 = src[1];
-            }
-            float4 *dst = reinterpret_cast<float4 *>(As + row * A_STRIDE + seg);
-            dst[0] = v0;
-            dst[1] = v1;
+            float4 *dst = reinterpret_cast<float4 *>(As + arow * A_STRIDE + aseg);
+            dst[0] = x0;
+            dst[1] = x1;
         }
         __syncthreads();
-
-        if (MODE == CTBB_FILTER_F32) {
+        // prefetch the next K chunk of this thread's row while the MFMAs run
+        if (k0 + BK < n && row_ok) {
+            x0 = reinterpret_cast<const float4 *>(xrow + k0 + BK)[0];
+            x1 = reinterpret_cast<const float4 *>(xrow + k0 + BK)[1];
+        }
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 4) {
-                float af[2], bfv[2];
+        for (int kk = 0; kk < BK; kk += 4) {
+            float af[2], bfv[2];
 #pragma unroll
-                for (int rt = 0; rt < 2; rt++) af[rt] = As[(wr * 32 + rt * 16 + lr) * A_STRIDE + kk + lk];
+            for (int rt = 0; rt < 2; rt++) af[rt] = As[(wr * 32 + rt * 16 + lr) * A_STRIDE + kk + lk];
 #pragma unroll
-                for (int ct = 0; ct < 2; ct++) {
-                    const int col = n0 + wc * 32 + ct * 16 + lr;
-                    bfv[ct] = Tf[n + col - (k0 + kk + lk)];
-                }
-#pragma unroll
-                for (int rt = 0; rt < 2; rt++)
-#pragma unroll
-                    for (int ct = 0; ct < 2; ct++)
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rt], bfv[ct], acc[rt][ct], 0, 0, 0);
+            for (int ct = 0; ct < 2; ct++) {
+                const int col = n0 + wc * 32 + ct * 16 + lr;
+                bfv[ct] = Tf[n + col - (k0 + kk + lk)];
             }
-        } else {
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 16) {
-                i16x4 ah[2], al[2], bh[2], bl[2];
+            for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-                for (int rt = 0; rt < 2; rt++) {
-                    const float4 x4 = *reinterpret_cast<const float4 *>(As + (wr * 32 + rt * 16 + lr) * A_STRIDE + kk + 4 * lk);
-                    const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        uint16_t h, l;
-                        if (MODE == CTBB_FILTER_BF16X3) {
-                            h = f32_to_bf16_rne(xs[e]);
-                            l = f32_to_bf16_rne(xs[e] - bf16_to_f32(h));
-                        } else {
-                            _Float16 hh = static_cast<_Float16>(xs[e]);
-                            _Float16 ll = static_cast<_Float16>(xs[e] - static_cast<float>(hh));
-                            h = __builtin_bit_cast(uint16_t, hh);
-                            l = __builtin_bit_cast(uint16_t, ll);
-                        }
-                        ah[rt][e] = static_cast<short>(h);
-                        al[rt][e] = static_cast<short>(l);
-                    }
-                }
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++) {
-                    const int col = n0 + wc * 32 + ct * 16 + lr;
-                    const int base = n + col - (k0 + kk + 4 * lk);
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        bh[ct][e] = static_cast<short>(Th[base - e]);
-                        bl[ct][e] = static_cast<short>(Tl[base - e]);
-                    }
-                }
-#pragma unroll
-                for (int rt = 0; rt < 2; rt++)
-#pragma unroll
-                    for (int ct = 0; ct < 2; ct++) {
-                        if (MODE == CTBB_FILTER_BF16X3) {
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rt], bl[ct], acc[rt][ct], 0, 0, 0);
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rt], bh[ct], acc[rt][ct], 0, 0, 0);
-                        } else {
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
-                                __builtin_bit_cast(f16x4, al[rt]), __builtin_bit_cast(f16x4, bh[ct]), acc[rt][ct], 0, 0, 0);
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
-                                __builtin_bit_cast(f16x4, ah[rt]), __builtin_bit_cast(f16x4, bl[ct]), acc[rt][ct], 0, 0, 0);
-                            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
-                                __builtin_bit_cast(f16x4, ah[rt]), __builtin_bit_cast(f16x4, bh[ct]), acc[rt][ct], 0, 0, 0);
-                        }
-                    }
-            }
+                for (int ct = 0; ct < 2; ct++)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rt], bfv[ct], acc[rt][ct], 0, 0, 0);
         }
     }
 
     // D layout of 16x16 MFMA tiles: lane l, register e -> row 4*(l/16)+e, column l%16
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const size_t om = m0 + wr * 32 + rt * 16 + 4 * lk + e;
+                const int col = n0 + wc * 32 + ct * 16 + lr;
+                if (om < a.rows) {
+                    const float y = acc[rt][ct][e];
+                    if (a.out_bf16)
+                        reinterpret_cast<uint16_t *>(a.out)[om * n + col] = f32_to_bf16_rne(y);
+                    else
+                        reinterpret_cast<float *>(a.out)[om * n + col] = y;
+                }
+            }
+}
+
+// Split-precision Toeplitz product: X = Xh + Xl and T = Th + Tl in bfloat16
+// (MODE BF16X3) or float16 (MODE F16X3, taps scaled by 2^10), accumulated
+// in FP32 as Xl*Th + Xh*Tl + Xh*Th on v_mfma_f32_16x16x16_{bf16,f16}.
+// A tile: split once into hi/lo 16-bit arrays in LDS (row stride 40).
+// B fragments: lane (col, k..k+3) needs T[n+col-k-e] = g[n-1-col+k+e] with
+// g the reversed taps; four copies of the reversed window shifted by 0..3
+// make every fragment one 8-byte-aligned LDS read.
+template <int MODE>
+__global__ __launch_bounds__(256) void toeplitz_split_kernel(FilterArgs a) {
+    constexpr int AS = BK + 8;                 // A row stride in 16-bit elements
+    extern __shared__ uint16_t sh[];
+    const int n = a.n;
+    const int tw = n + 72;                     // window length per copy
+    uint16_t *Ah = sh, *Al = sh + BM * AS;
+    uint16_t *Gh = Al + BM * AS;               // [4][tw]
+    uint16_t *Gl = Gh + 4 * tw;                // [4][tw]
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    const size_t m0 = static_cast<size_t>(blockIdx.y) * BM;
+    const int n0 = blockIdx.x * BN;
+    const int jb = n - 1 - (n0 + BN - 1);      // first reversed-tap index this tile needs
+    const float tscale = MODE == CTBB_FILTER_F16X3 ? 1024.0f : 1.0f;
+
+    auto split = [&](float v, uint16_t &h, uint16_t &l) {
+        if (MODE == CTBB_FILTER_BF16X3) {
+            const __bf16 hb = static_cast<__bf16>(v);
+            h = __builtin_bit_cast(uint16_t, hb);
+            l = __builtin_bit_cast(uint16_t, static_cast<__bf16>(v - static_cast<float>(hb)));
+        } else {
+            const _Float16 hf = static_cast<_Float16>(v);
+            h = __builtin_bit_cast(uint16_t, hf);
+            l = __builtin_bit_cast(uint16_t, static_cast<_Float16>(v - static_cast<float>(hf)));
+        }
+    };
+
+    for (int e = tid; e < 4 * tw; e += 256) {
+        const int s = e / tw, m = e - s * tw;
+        const int j = jb + m + s;             // reversed-tap index
+        const float t = (j >= 0 && j < 2 * n) ? a.taps[2 * n - 1 - j] * tscale : 0.0f;
+        split(t, Gh[e], Gl[e]);
+    }
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int arow = tid >> 2, aseg = (tid & 3) * 8;
+    const size_t gm = m0 + arow;
+    const bool row_ok = gm < a.rows;
+    const float *xrow = a.in + gm * n + aseg;
+    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
+    if (row_ok) {
+        x0 = reinterpret_cast<const float4 *>(xrow)[0];
+        x1 = reinterpret_cast<const float4 *>(xrow)[1];
+    }
+
+    for (int k0 = 0; k0 < n; k0 += BK) {
+        __syncthreads();   // previous tile consumed (and taps written, first pass)
+        {
+            const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            uint16_t h[8], l[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) split(xs[e], h[e], l[e]);
+            i16x4 h0 = {(short)h[0], (short)h[1], (short)h[2], (short)h[3]};
+            i16x4 h1 = {(short)h[4], (short)h[5], (short)h[6], (short)h[7]};
+            i16x4 l0 = {(short)l[0], (short)l[1], (short)l[2], (short)l[3]};
+            i16x4 l1 = {(short)l[4], (short)l[5], (short)l[6], (short)l[7]};
+            i16x4 *ph = reinterpret_cast<i16x4 *>(Ah + arow * AS + aseg);
+            i16x4 *pl = reinterpret_cast<i16x4 *>(Al + arow * AS + aseg);
+            ph[0] = h0; ph[1] = h1;
+            pl[0] = l0; pl[1] = l1;
+        }
+        __syncthreads();
+        // prefetch the next K chunk of this thread's row while the MFMAs run
+        if (k0 + BK < n && row_ok) {
+            x0 = reinterpret_cast<const float4 *>(xrow + k0 + BK)[0];
+            x1 = reinterpret_cast<const float4 *>(xrow + k0 + BK)[1];
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            i16x4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                const int row = wr * 32 + rt * 16 + lr;
+                ah[rt] = *reinterpret_cast<const i16x4 *>(Ah + row * AS + kk + 4 * lk);
+                al[rt] = *reinterpret_cast<const i16x4 *>(Al + row * AS + kk + 4 * lk);
+            }
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++) {
+                const int col = n0 + wc * 32 + ct * 16 + lr;
+                const int mm = (n - 1 - col + k0 + kk + 4 * lk) - jb;
+                const int sft = mm & 3;
+                bh[ct] = *reinterpret_cast<const i16x4 *>(Gh + sft * tw + (mm - sft));
+                bl[ct] = *reinterpret_cast<const i16x4 *>(Gl + sft * tw + (mm - sft));
+            }
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) {
+                    if (MODE == CTBB_FILTER_BF16X3) {
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rt], bl[ct], acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rt], bh[ct], acc[rt][ct], 0, 0, 0);
+                    } else {
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
+                            __builtin_bit_cast(f16x4, al[rt]), __builtin_bit_cast(f16x4, bh[ct]), acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
+                            __builtin_bit_cast(f16x4, ah[rt]), __builtin_bit_cast(f16x4, bl[ct]), acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
+                            __builtin_bit_cast(f16x4, ah[rt]), __builtin_bit_cast(f16x4, bh[ct]), acc[rt][ct], 0, 0, 0);
+                    }
+                }
+        }
+    }
+
     const float oscale = 1.0f / tscale;
 #pragma unroll
     for (int rt = 0; rt < 2; rt++)
@@ -180,14 +249,14 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(FilterArgs a) {
         for (int ct = 0; ct < 2; ct++)
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const size_t gm = m0 + wr * 32 + rt * 16 + 4 * lk + e;
+                const size_t om = m0 + wr * 32 + rt * 16 + 4 * lk + e;
                 const int col = n0 + wc * 32 + ct * 16 + lr;
-                if (gm < a.rows) {
+                if (om < a.rows) {
                     const float y = acc[rt][ct][e] * oscale;
                     if (a.out_bf16)
-                        reinterpret_cast<uint16_t *>(a.out)[gm * n + col] = f32_to_bf16_rne(y);
+                        reinterpret_cast<uint16_t *>(a.out)[om * n + col] = f32_to_bf16_rne(y);
                     else
-                        reinterpret_cast<float *>(a.out)[gm * n + col] = y;
+                        reinterpret_cast<float *>(a.out)[om * n + col] = y;
                 }
             }
 }
@@ -227,8 +296,9 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
         return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
     }
     const size_t gy = (a.rows + BM - 1) / BM;
-    const size_t lds = BM * A_STRIDE * sizeof(float) +
-                       (a.mode == CTBB_FILTER_F32 ? 2 * a.n * sizeof(float) : 4 * a.n * sizeof(uint16_t));
+    const size_t lds = a.mode == CTBB_FILTER_F32
+                           ? BM * A_STRIDE * sizeof(float) + 2 * a.n * sizeof(float)
+                           : (2 * BM * (BK + 8) + 8 * (static_cast<size_t>(a.n) + 72)) * sizeof(uint16_t);
     if (lds > 64 * 1024) return CTBB_ERR_SHAPE;
     // grid.y may exceed 65535 for very tall inputs: split into launches
     const size_t max_y = 65535;
@@ -241,9 +311,9 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
         s.rows = a.rows - y0 * BM < ny * BM ? a.rows - y0 * BM : ny * BM;
         dim3 grid(a.n / BN, static_cast<unsigned>(ny));
         switch (a.mode) {
-            case CTBB_FILTER_F32: hipLaunchKernelGGL(toeplitz_kernel<CTBB_FILTER_F32>, grid, dim3(256), lds, stream, s); break;
-            case CTBB_FILTER_BF16X3: hipLaunchKernelGGL(toeplitz_kernel<CTBB_FILTER_BF16X3>, grid, dim3(256), lds, stream, s); break;
-            case CTBB_FILTER_F16X3: hipLaunchKernelGGL(toeplitz_kernel<CTBB_FILTER_F16X3>, grid, dim3(256), lds, stream, s); break;
+            case CTBB_FILTER_F32: hipLaunchKernelGGL(toeplitz_f32_kernel, grid, dim3(256), lds, stream, s); break;
+            case CTBB_FILTER_BF16X3: hipLaunchKernelGGL(toeplitz_split_kernel<CTBB_FILTER_BF16X3>, grid, dim3(256), lds, stream, s); break;
+            case CTBB_FILTER_F16X3: hipLaunchKernelGGL(toeplitz_split_kernel<CTBB_FILTER_F16X3>, grid, dim3(256), lds, stream, s); break;
             default: return CTBB_ERR_ARG;
         }
         if (hipGetLastError() != hipSuccess) return CTBB_ERR_HIP;

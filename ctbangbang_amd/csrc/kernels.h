@@ -20,7 +20,7 @@ struct BpArgs {
     const BpView *vtab;        // [n_ht*half]
     float *out;                // [32][nx][ny]
     int nx, ny, nos, n_rows, half, n_ht;
-    int wch_log2;              // staged channel window = 1 << wch_log2
+    int tpr, tpr_log2;         // staging threads per detector row (set by the launcher)
     int nz;                    // slices per thread: 32 (one z chunk) or 16 (two)
     int tiles_y;               // ny / 16
     int n_tiles;

@@ -75,6 +75,28 @@ def test_ramp_filter(ctx, mode, tol, n):
     ctx.set_options(filter_mode=cb.FILTER_F32)
 
 
+@pytest.mark.parametrize("mode", [cb.FILTER_F32, cb.FILTER_BF16X3])
+def test_ramp_filter_bf16_output(ctx, mode):
+    """bfloat16 storage of the filtered rows: round-to-nearest-even of the
+    FP32-accumulated result (<= 2^-8 relative per sample, plus the filter's
+    own absolute error bound of 3e-5 * max)."""
+    import oracle
+    n, rows = 1472, 200
+    rng = np.random.default_rng(77)
+    x = rng.standard_normal((rows, n)).astype(np.float32)
+    taps = cb.ramp_filter(n, 0.35, 1.0, 1.0)
+    want = oracle.filter_fft(x, taps)
+    ctx.set_options(filter_mode=mode, proj_dtype=cb.PROJ_BF16)
+    out = torch.empty(rows * n, dtype=torch.int16, device="cuda")
+    ctx.filter_rows_dev(dev(taps), n, rows, dev(x), out)
+    ctx.set_options(filter_mode=cb.FILTER_F32, proj_dtype=cb.PROJ_F32)
+    bits = host(out).astype(np.uint16).astype(np.uint32) << 16
+    got = bits.view(np.float32).reshape(rows, n)
+    excess = np.abs(got - want) - 2.0 ** -8 * np.abs(want)
+    print(f"filter bf16 out mode={mode}: max excess over 2^-8 rel {excess.max() / np.abs(want).max():.3g} of max")
+    assert excess.max() <= 3e-5 * np.abs(want).max()
+
+
 def test_filter_impulse_gives_taps(ctx):
     n = 1472
     taps = np.fromfile(os.path.join(cb.PKG_DIR, "resources", "filters", "f_736_sharp.txt"), dtype=np.float32)
@@ -256,6 +278,32 @@ def test_end_to_end_hu(ctx, tmp_path, mini_dir, ffs, table_dir):
     rms, mx = float(np.sqrt(np.mean(d ** 2))), float(np.abs(d).max())
     print(f"end-to-end {ffs}/{table_dir}: HU rms {rms:.4f} max {mx:.4f}; image HU range {want.min():.0f}..{want.max():.0f}")
     assert rms <= 1.0 and mx <= 10.0
+    cb.free_metadata(mr)
+
+
+@pytest.mark.parametrize("mode,proj", [(cb.FILTER_BF16X3, cb.PROJ_F32), (cb.FILTER_F16X3, cb.PROJ_F32),
+                                       (cb.FILTER_BF16X3, cb.PROJ_BF16)])
+def test_end_to_end_precision_form_hu(ctx, tmp_path, mini_dir, mode, proj):
+    """Precision form (split MFMA filter, optionally BF16 projections with FP32
+    accumulation) vs the FP64 oracle: the north-star bar is <= 1 HU RMS."""
+    tok = scans.mini_tokens("both", nrows=8, nx=32)
+    mr = _mr(tmp_path, tok, mini_dir)
+    pp = scans.phantom_for(tok, 5)
+    n_read = tok["Readings"]
+    frame = mr.cg.n_rows_raw * mr.cg.n_channels
+    raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda")
+    ctx.synth_scan_dev(mr, pp, 0, n_read, raw)
+    s = cb.block_sizes(mr)
+    out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=mode, proj_dtype=proj)
+    ctx.reconstruct_dev(mr, raw, n_read, out)
+    ctx.set_options(filter_mode=cb.FILTER_F32, proj_dtype=cb.PROJ_F32)
+    got = host(out).reshape(-1, mr.rp.nx, mr.rp.ny)
+    want = _oracle_volume(mr, host(raw).reshape(n_read, frame), cb.load_filter(mr), True, 0.02)
+    d = got.astype(np.float64) - want
+    rms, mx = float(np.sqrt(np.mean(d ** 2))), float(np.abs(d).max())
+    print(f"precision form mode={mode} proj={proj}: HU rms {rms:.4f} max {mx:.4f}")
+    assert rms <= 1.0
     cb.free_metadata(mr)
 
 
