@@ -4,8 +4,10 @@ blocks -> 40 slices of 3 mm in HU) with the scan resident in HBM.
 
 One step = one complete reconstruction: per block adaptive filter, rebin,
 ramp filter and backprojection, then reorder + thickening + HU.
-Prints one JSON line (see keys below).  --form mixed selects the precision
-form (BF16 hi+lo MFMA filter, BF16 filtered projections, FP32 accumulation).
+Prints one JSON line (see keys below).  --form plain computes in FP32
+throughout (ramp filter as an FP32 FFT convolution by default); --form mixed
+selects the precision form (FP16 hi+lo MFMA Toeplitz filter, BF16 filtered
+projections, FP32 accumulation).
 """
 import argparse
 import json
@@ -35,6 +37,8 @@ def parse():
                     help="all: whole volume (the benchmark); backproject: only the backprojection of one block "
                          "(for counter runs); copy: a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--bp-tile", type=int, default=32, choices=[16, 32])
+    ap.add_argument("--filter", default=None, choices=["fft32", "f32", "bf16x3", "f16x3"],
+                    help="ramp filter form (default: fft32 for --form plain, f16x3 for --form mixed)")
     return ap.parse_args()
 
 
@@ -137,7 +141,9 @@ def main():
     mr = cb.metadata_from_prm(prm)
     ctx = cb.Context(0)
     mixed = args.form == "mixed"
-    ctx.set_options(filter_mode=cb.FILTER_F16X3 if mixed else cb.FILTER_F32,
+    modes = {"fft32": cb.FILTER_FFT32, "f32": cb.FILTER_F32, "bf16x3": cb.FILTER_BF16X3, "f16x3": cb.FILTER_F16X3}
+    filt = args.filter or ("f16x3" if mixed else "fft32")
+    ctx.set_options(filter_mode=modes[filt],
                     proj_dtype=cb.PROJ_BF16 if mixed else cb.PROJ_F32,
                     mu_water=float(tok.get("MuWater", 0.0)),
                     adaptive_filter=1 if float(tok["AdaptiveFiltration"]) > 0 else 0,
@@ -203,7 +209,7 @@ def main():
         "config": {"workload": f"{args.config}: {int(mr.rp.nx)}^2 x {int(mr.ri.n_slices_recon)} raw slices, "
                                f"{int(mr.ri.n_blocks)} blocks, n_ffs={int(mr.ri.n_ffs)}, "
                                f"{int(mr.cg.n_rows)} rows, {n_read} readings",
-                   "form": args.form, "voxel_updates": vu_total, "parallelism": "1 GPU"},
+                   "form": args.form, "filter": filt, "voxel_updates": vu_total, "parallelism": "1 GPU"},
         "backproject_updates_per_s": vu_launch / bp_avg_s,
         "kernels_last_step": fam,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -25,12 +25,12 @@ __all__ = [
     "CTBBError", "lib", "library_path", "ReconMetadata", "ReconParams", "CtGeom", "ReconInfo", "Options",
     "PhantomParams", "Sizes", "Context", "write_prm", "configure_recon_params", "configure_ct_geom",
     "plan_reconstruction", "plan_block", "block_sizes", "n_slices_final", "load_filter", "ramp_filter",
-    "metadata_from_prm", "synth_scan", "free_metadata", "FILTER_F32", "FILTER_BF16X3", "FILTER_F16X3",
+    "metadata_from_prm", "synth_scan", "free_metadata", "FILTER_F32", "FILTER_BF16X3", "FILTER_F16X3", "FILTER_FFT32",
     "PROJ_F32", "PROJ_BF16", "adaptive_filter_kk", "rebin_filter", "backproject", "finalize_image_stack",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-FILTER_F32, FILTER_BF16X3, FILTER_F16X3 = 0, 1, 2
+FILTER_F32, FILTER_BF16X3, FILTER_F16X3, FILTER_FFT32 = 0, 1, 2, 3
 PROJ_F32, PROJ_BF16 = 0, 1
 
 _STATUS = {

@@ -21,9 +21,11 @@
 // the reference's border addressing.
 //
 // W(q) = 0 for |q| >= 1 contributes exactly nothing; since q varies by less
-// than 1% across a wavefront, slices no lane sees are skipped on a
-// wave-uniform vote, and the cos^2 taper is evaluated only when some lane is
-// in 0.6 <= |q| < 1.
+// than 1% across a wavefront, chunks of 4 slices that no lane sees are
+// skipped on a wave-uniform vote (about 40% of the (slice, view) pairs are
+// seen at the C1 geometry), and the cos^2 taper is evaluated only for chunks
+// where some lane is in 0.6 <= |q| < 1.  View-table entries reach the
+// workgroup through a small LDS ring refilled every 64 views.
 //
 // Workgroups are remapped so the tiles running on one XCD are spatially
 // contiguous: their footprints on each view overlap and hit in that XCD's L2.
@@ -35,26 +37,21 @@ namespace ctbb {
 
 namespace {
 
-// cos^2(pi'/2 * t) on t in [0, 1] (pi' = 3.1415368979f, the constant the
-// reference's weight uses): degree-8 Chebyshev interpolant converted to a
-// power series, max abs error 7e-7 in FP32 Horner form.
-__device__ __forceinline__ float taper(float t) {
-    float p = 3.0801288630755153e-06f;
-    p = fmaf(p, t, 0.27724477648735046f);
-    p = fmaf(p, t, -0.970396876335144f);
-    p = fmaf(p, t, 0.18480874598026276f);
-    p = fmaf(p, t, 1.9640635251998901f);
-    p = fmaf(p, t, 0.012797481380403042f);
-    p = fmaf(p, t, -2.468567371368408f);
-    p = fmaf(p, t, 4.714697934105061e-05f);
-    p = fmaf(p, t, 0.9999997019767761f);
-    return p;
-}
-
 // Taper part of the Stierstorfer weight W(q), Q = 0.6 (reference
-// include/backproject.cuh:33-46), for 0.6 <= |q| < 1.
+// include/backproject.cuh:33-46): cos^2(pi'/2 * t), t = (|q| - 0.6) / 0.4,
+// pi' = 3.1415368979f, for 0.6 <= |q| < 1.  Degree-7 polynomial in t with
+// p(0) = 1 exactly, fitted in FP64 (max abs error 5.8e-7 in FP32 Horner
+// form); the result is clamped to [0, 1] so weights are never negative.
 __device__ __forceinline__ float stier_taper(float aq) {
-    return taper(__builtin_amdgcn_fmed3f((aq - 0.6f) * 2.5f, 0.0f, 1.0f));
+    const float t = __builtin_amdgcn_fmed3f(fmaf(aq, 2.5f, -1.5f), 0.0f, 1.0f);
+    float p = 0.27661213278770447f;
+    p = fmaf(p, t, -0.968159556388855f);
+    p = fmaf(p, t, 0.181683748960495f);
+    p = fmaf(p, t, 1.9662673473358154f);
+    p = fmaf(p, t, 0.01197277195751667f);
+    p = fmaf(p, t, -2.4684107303619385f);
+    p = fmaf(p, t, 3.430374272284098e-05f);
+    return __builtin_amdgcn_fmed3f(fmaf(p, t, 1.0f), 0.0f, 1.0f);
 }
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
@@ -89,17 +86,26 @@ __device__ __forceinline__ float asin_small(float x) {
     return p * x;
 }
 
-// NZ slices per thread; CPT (even) staged channels per thread per view.  The
+// NZ slices per thread; CPT staged channels per thread per view; TPR staging
+// threads per detector row (0: taken from the arguments at run time).  The
 // staged window covers rows_pad x W channels, rows_pad = 256 / TPR a power of
 // two >= n_rows, W = TPR * CPT: thread t stages row t / TPR, channels
 // (t % TPR) * CPT ... + CPT - 1 from CPT + 1 consecutive samples of that row.
-// In LDS every run of CPT pairs is followed by one pad pair, so channel c of
-// row r sits at r * TPR * (CPT + 1) + c + c / CPT: the lanes of a wave then
-// write at a stride of CPT + 1 pairs (odd), which spreads them over all banks.
-template <int NZ, int CPT, bool kBf16>
-__global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
-    static_assert(CPT % 2 == 0, "CPT must be even");
-    extern __shared__ float2 lds2[];   // 2 buffers x [rows_pad][TPR * (CPT + 1)] (value, delta)
+// LDS holds (value, delta) pairs row-major; every run of CPT pairs is
+// followed by one pad pair, so channel c of row r sits at r * RS + c + c / CPT
+// with RS = TPR * (CPT + 1): a wave's staging writes land at a stride of
+// CPT + 1 pairs, and its bilinear reads of neighbouring columns at
+// neighbouring channels.  Row rows_pad is kept zero so a row coordinate that
+// rounds to exactly n_rows - 1 reads a finite neighbour (its weight is 0).
+static_assert(sizeof(BpView) == 16 && alignof(BpView) <= 16, "view table entries are read as float4");
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
+template <int NZ, int CPT, int TPR, bool kBf16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bp_kernel(BpArgs a) {
+    // dynamic LDS: 2 window buffers x [rows_pad + 1][RS] pairs, then a ring
+    // of 128 view-table entries
+    extern __shared__ float2 lds2[];
     const int tid = threadIdx.x;
 
     // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give each
@@ -120,20 +126,27 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const float xm = a.xs * (static_cast<float>(tx * 16) + 7.5f - a.xc) + a.x0;
     const float ym = a.ys * (static_cast<float>(ty * 16) + 7.5f - a.yc) + a.y0;
 
-    const int W = a.tpr * CPT;                          // staged channels
-    const int row_pairs = a.tpr * (CPT + 1);            // LDS pairs per staged row
-    const int n_stage = 256 * (CPT + 1);                // pairs per buffer
-    const unsigned stride_b = static_cast<unsigned>(row_pairs) * 8u;
-    const int n_views = a.half * a.n_ht;
-    const int srow = tid >> a.tpr_log2;                 // staged row of this thread
-    const int scol = (tid & (a.tpr - 1)) * CPT;         // first staged channel
+    const int tpr = TPR ? TPR : a.tpr;
+    const int tpr_log2 = TPR ? ilog2(TPR) : a.tpr_log2;
+    const int rows_pad = 256 >> tpr_log2;
+    const int W = tpr * CPT;                            // staged channels
+    const int RS = tpr * (CPT + 1);                     // pairs per staged row
+    const int n_stage = (rows_pad + 1) * RS;            // pairs per buffer
+    const int srow = tid >> tpr_log2;                   // staged row of this thread
+    const int scol = (tid & (tpr - 1)) * CPT;           // first staged channel
     const bool srow_ok = srow < a.n_rows;
+
+    // the zero row (row rows_pad) of both buffers
+    for (int c = tid; c < RS; c += 256) {
+        lds2[rows_pad * RS + c] = make_float2(0.0f, 0.0f);
+        lds2[n_stage + rows_pad * RS + c] = make_float2(0.0f, 0.0f);
+    }
 
     // prefetch registers for the next view's window (raw sample bits)
     uint32_t pv[CPT + 1];
 
-    auto window_lo = [&](const BpView &vw) {
-        const float pm = (xm * vw.sin_t - ym * vw.cos_t) * a.inv_ds + a.two_cc;
+    auto window_lo = [&](const float4 vw) {
+        const float pm = (xm * vw.x - ym * vw.y) * a.inv_ds + a.two_cc;
         return static_cast<int>(floorf(pm - a.half_w)) - 1;
     };
     auto fetch = [&](int v, int p_lo) {
@@ -142,7 +155,13 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
         if (srow_ok && p_lo >= 0 && p_lo + W < a.nos) {
             // window inside the detector: CPT + 1 consecutive samples
 #pragma unroll
-            for (int e = 0; e <= CPT; e++) pv[e] = load_bits<kBf16>(a, rb + g0 + e);
+            for (int e = 0; e <= CPT; e++) {
+                pv[e] = load_bits<kBf16>(a, rb + g0 + e);
+                // keep the loads separate: merged multi-dword loads land in
+                // register tuples that the (value, delta) pairs cannot use,
+                // and the copies out of them would wait for the loads here
+                asm volatile("" ::: "memory");
+            }
         } else {
 #pragma unroll
             for (int e = 0; e <= CPT; e++) {
@@ -156,18 +175,35 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
 #pragma unroll
     for (int z = 0; z < NZ; z++) { s[z] = 0.0f; st[z] = 0.0f; ht[z] = 0.0f; }
 
-    // pipeline prologue: view 0 (i = 0, k = 0)
-    BpView vw_next = a.vtab[0];
-    int plo_next = window_lo(vw_next);
-    fetch(0, plo_next);
+    // Views run in reference order t = i * n_ht + k (angle i, half turn k);
+    // the table holds them in that order.  Pipeline: while view t is
+    // computed, the window of view t + 1 is in flight.  Table entries pass
+    // through an LDS ring of 128: chunk c (views 64c .. 64c + 63) is loaded
+    // during view 64(c - 1), published by the next barrier and first read
+    // 63 views later; its slot last held chunk c - 2.
+    const int n_views = a.half * a.n_ht;
+    const float4 *vtab = reinterpret_cast<const float4 *>(a.vtab);
+    float4 *ring = reinterpret_cast<float4 *>(lds2 + 2 * n_stage);
+    auto load_chunk = [&](int c) {
+        if (tid < 64) ring[(c & 1) * 64 + tid] = vtab[min(64 * c + tid, n_views - 1)];
+    };
+    load_chunk(0);
+    __syncthreads();
+    int plo_next;
+    {
+        const float4 v0 = ring[0];
+        plo_next = window_lo(v0);
+        fetch(__float_as_int(v0.w), plo_next);
+    }
 
+    int k = 0;                                          // half turn of view t
     for (int t = 0; t < n_views; t++) {
-        const int i = t / a.n_ht, k = t - i * a.n_ht;
-        const BpView vw = vw_next;
+        const bool last_k = k == a.n_ht - 1;            // view t closes its angle
+        if (++k == a.n_ht) k = 0;
         const int p_lo = plo_next;
         float2 *buf = lds2 + (t & 1) * n_stage;
         {
-            float2 *dst = buf + tid * (CPT + 1);   // = srow * row_pairs + scol + scol / CPT
+            float2 *dst = buf + tid * (CPT + 1);        // = srow * RS + scol + scol / CPT
             float prev = bits_to_f32<kBf16>(pv[0]);
 #pragma unroll
             for (int e = 0; e < CPT; e++) {
@@ -177,16 +213,17 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
             }
         }
         __syncthreads();
+        if ((t & 63) == 0 && t + 64 < n_views) load_chunk((t >> 6) + 1);
+        const float4 vw = ring[t & 127];
         if (t + 1 < n_views) {
-            const int i1 = (t + 1) / a.n_ht, k1 = (t + 1) - i1 * a.n_ht;
-            vw_next = a.vtab[i1 + k1 * a.half];
-            plo_next = window_lo(vw_next);
-            fetch(i1 + k1 * a.half, plo_next);
+            const float4 vn = ring[(t + 1) & 127];
+            plo_next = window_lo(vn);
+            fetch(__float_as_int(vn.w), plo_next);
         }
 
         // per-column geometry, shared by the NZ slices:
         // q(z) = A z + B is qhat of reference include/backproject.cuh:84
-        const float phat = x * vw.sin_t - y * vw.cos_t;
+        const float phat = x * vw.x - y * vw.y;
         const float pl = phat * a.inv_ds + a.two_cc - static_cast<float>(p_lo);
         int c0 = static_cast<int>(floorf(pl));
         const float fp = pl - static_cast<float>(c0);
@@ -194,50 +231,71 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
         const float xr = phat * a.inv_rf;
         float asn = asin_small(xr);
         if (__any(fabsf(xr) > 0.45f)) asn = fabsf(xr) > 0.45f ? asinf(xr) : asn;
-        const float dz0 = vw.c_v + a.k_asin * asn + static_cast<float>(z0) * a.coll;
-        const float lhat = sqrtf(a.rf2 - phat * phat) - x * vw.cos_t - y * vw.sin_t;
+        const float dz0 = vw.z + a.k_asin * asn + static_cast<float>(z0) * a.coll;
+        const float lhat = sqrtf(a.rf2 - phat * phat) - x * vw.y - y * vw.x;
         const float g = a.neg_td * __builtin_amdgcn_rcpf(lhat * a.tanc);
         const float A = g * a.coll;
         const float B = g * dz0;
-        const float Ar = A * a.hr, Br = fmaf(B, a.hr, a.hr);   // row coordinate = Ar z + Br
-        const unsigned cl = static_cast<unsigned>(c0) + static_cast<unsigned>(c0) / CPT;
-        const char *colb = reinterpret_cast<const char *>(buf) + 8u * cl;
+        // LDS index of (row 0, channel c0) in this view's buffer
+        const unsigned colb = static_cast<unsigned>((t & 1) * n_stage) + static_cast<unsigned>(c0) +
+                              static_cast<unsigned>(c0) / CPT;
 
+        // Slices in chunks of 4: one wave vote per chunk skips chunks no lane
+        // sees; the 4 slices of a visible chunk run without branches (W = 0 for
+        // lanes that do not see a slice, row coordinate clamped into the
+        // window), so their LDS reads are in flight together.  The taper is
+        // evaluated only for chunks where some lane has 0.6 <= |q| < 1.
+        const float two_hr = 2.0f * a.hr;
 #pragma unroll
-        for (int z = 0; z < NZ; z++) {
-            const float q = fmaf(A, static_cast<float>(z), B);
-            const float aq = fabsf(q);
-            const bool zv = aq < 1.0f;
-            if (__any(zv)) {
-                float w = zv ? 1.0f : 0.0f;
-                const bool tap = zv && aq >= 0.6f;
-                if (__any(tap)) {
-                    const float tw = stier_taper(aq);
-                    w = tap ? tw : w;
+        for (int zc = 0; zc < NZ; zc += 4) {
+            float q[4], w[4];
+            unsigned long long mv = 0, mt = 0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                q[c] = fmaf(A, static_cast<float>(zc + c), B);
+                const float aq = fabsf(q[c]);
+                mv |= __builtin_amdgcn_ballot_w64(aq < 1.0f);
+                mt |= __builtin_amdgcn_ballot_w64(aq < 1.0f && aq >= 0.6f);
+            }
+            if (mv == 0) continue;
+            if (mt != 0) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const float aq = fabsf(q[c]);
+                    const float tw = aq >= 0.6f ? stier_taper(aq) : 1.0f;
+                    w[c] = aq < 1.0f ? tw : 0.0f;
                 }
-                const float rowf = __builtin_amdgcn_fmed3f(fmaf(Ar, static_cast<float>(z), Br), 0.0f, a.row_max);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) w[c] = fabsf(q[c]) < 1.0f ? 1.0f : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                // row coordinate (q + 1) hr, in [0, n_rows - 1] where W != 0
+                const float rowf = __builtin_amdgcn_fmed3f(fmaf(q[c], a.hr, a.hr), 0.0f, two_hr);
                 const unsigned r0 = static_cast<unsigned>(rowf);
                 const float fr = __builtin_amdgcn_fractf(rowf);
-                const char *p = colb + __umul24(r0, stride_b);
-                const float2 e0 = *reinterpret_cast<const float2 *>(p);
-                const float2 e1 = *reinterpret_cast<const float2 *>(p + stride_b);
+                const unsigned idx = colb + __umul24(r0, static_cast<unsigned>(RS));
+                const float2 e0 = lds2[idx];
+                const float2 e1 = lds2[idx + RS];
                 const float top = fmaf(fp, e0.y, e0.x);
                 const float bot = fmaf(fp, e1.y, e1.x);
                 const float val = fmaf(fr, bot - top, top);
-                st[z] = fmaf(val, w, st[z]);
-                ht[z] += w;
+                st[zc + c] = fmaf(val, w[c], st[zc + c]);
+                ht[zc + c] += w[c];
             }
         }
-        if (k == a.n_ht - 1) {
-            // s += (1/h_t) s_t where h_t != 0 (reference include/backproject.cuh:104-108)
+        if (last_k) {
+            // s += (1/h_t) s_t where h_t != 0 (reference include/backproject.cuh:104-108).
+            // Weights are >= 0, so h_t = 0 implies s_t = 0 and the clamp
+            // turns that case into 0 * 1e30 = 0 without a select.
 #pragma unroll
             for (int z = 0; z < NZ; z++) {
-                s[z] += ht[z] != 0.0f ? __builtin_amdgcn_rcpf(ht[z]) * st[z] : 0.0f;
+                s[z] = fmaf(st[z], __builtin_amdgcn_rcpf(fmaxf(ht[z], 1e-30f)), s[z]);
                 st[z] = 0.0f;
                 ht[z] = 0.0f;
             }
         }
-        (void)i;
     }
 
     const bool in_fov = sqrtf(x * x + y * y) < a.acq_half;
@@ -247,21 +305,26 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     for (int z = 0; z < NZ; z++) o[z * nxy] = in_fov ? s[z] * a.out_scale : 0.0f;
 }
 
-template <int NZ, int CPT>
+template <int NZ, int CPT, int TPR>
 int launch_c(const BpArgs &a, hipStream_t stream) {
-    const size_t lds_bytes = 2 * 256 * (CPT + 1) * sizeof(float2);   // <= 60 KiB
+    const int rows_pad = 256 >> a.tpr_log2;
+    const size_t lds_bytes = 2 * static_cast<size_t>(rows_pad + 1) * a.tpr * (CPT + 1) * sizeof(float2) +
+                             128 * sizeof(float4);
+    if (lds_bytes > 65536) return CTBB_ERR_SHAPE;
     dim3 grid(a.n_tiles, 32 / NZ);
-    if (a.rebin_bf16) hipLaunchKernelGGL((bp_kernel<NZ, CPT, true>), grid, dim3(256), lds_bytes, stream, a);
-    else hipLaunchKernelGGL((bp_kernel<NZ, CPT, false>), grid, dim3(256), lds_bytes, stream, a);
+    if (a.rebin_bf16) hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, true>), grid, dim3(256), lds_bytes, stream, a);
+    else hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, false>), grid, dim3(256), lds_bytes, stream, a);
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 
-// Smallest supported channels-per-thread whose window W = tpr * CPT covers
-// the tile footprint (2 half_w + 3 channels).
+// Channels per thread for one slices-per-thread variant: the smallest
+// supported CPT whose window W = tpr * CPT covers the tile footprint
+// (2 half_w + 3 channels).  The (TPR, CPT) shapes of 16-, 32- and 64-row
+// detectors have kernels with compile-time strides; others use the generic one.
 template <int NZ>
 int launch_nz(BpArgs a, hipStream_t stream) {
     const float need = 2.0f * a.half_w + 3.0f;
-    int rows_pad = 1;
+    int rows_pad = 2;
     while (rows_pad < a.n_rows) rows_pad <<= 1;
     if (rows_pad > 256) return CTBB_ERR_SHAPE;
     a.tpr = 256 / rows_pad;
@@ -271,14 +334,30 @@ int launch_nz(BpArgs a, hipStream_t stream) {
     static const int kCpt[] = {2, 4, 6, 8, 10, 12, 14};
     for (int c : kCpt) {
         if (c < cpt) continue;
+        if constexpr (NZ == 32) {
+            const int key = a.tpr * 100 + c;
+            switch (key) {
+                case 406: return launch_c<NZ, 6, 4>(a, stream);
+                case 408: return launch_c<NZ, 8, 4>(a, stream);
+                case 410: return launch_c<NZ, 10, 4>(a, stream);
+                case 412: return launch_c<NZ, 12, 4>(a, stream);
+                case 414: return launch_c<NZ, 14, 4>(a, stream);
+                case 804: return launch_c<NZ, 4, 8>(a, stream);
+                case 806: return launch_c<NZ, 6, 8>(a, stream);
+                case 808: return launch_c<NZ, 8, 8>(a, stream);
+                case 1602: return launch_c<NZ, 2, 16>(a, stream);
+                case 1604: return launch_c<NZ, 4, 16>(a, stream);
+                default: break;
+            }
+        }
         switch (c) {
-            case 2: return launch_c<NZ, 2>(a, stream);
-            case 4: return launch_c<NZ, 4>(a, stream);
-            case 6: return launch_c<NZ, 6>(a, stream);
-            case 8: return launch_c<NZ, 8>(a, stream);
-            case 10: return launch_c<NZ, 10>(a, stream);
-            case 12: return launch_c<NZ, 12>(a, stream);
-            case 14: return launch_c<NZ, 14>(a, stream);
+            case 2: return launch_c<NZ, 2, 0>(a, stream);
+            case 4: return launch_c<NZ, 4, 0>(a, stream);
+            case 6: return launch_c<NZ, 6, 0>(a, stream);
+            case 8: return launch_c<NZ, 8, 0>(a, stream);
+            case 10: return launch_c<NZ, 10, 0>(a, stream);
+            case 12: return launch_c<NZ, 12, 0>(a, stream);
+            case 14: return launch_c<NZ, 14, 0>(a, stream);
         }
     }
     return CTBB_ERR_SHAPE;   // footprint wider than 14 channels per thread

@@ -22,12 +22,14 @@ void plan_block(recon_metadata *mr, int block_idx);
 constexpr float kPiBp = 3.1415368979f;
 constexpr float kPiTrue = 3.1415926535897f;
 
-// Per-view table entry of the backprojection (FP64-derived, stored FP32).
+// Per-view table entry of the backprojection (FP64-derived, stored FP32),
+// in the order the views are processed: entry t = i * n_half_turns + k for
+// angle i and half turn k (reference include/backproject.cuh:70-109 loops).
 struct BpView {
     float sin_t;    // sin(theta_v)
     float cos_t;    // cos(theta_v)
     float c_v;      // z_rot/2 + z_rot*(tube_start - theta_v)/(2 pi'): z - ray_pos at slice 0, column term excluded
-    float pad;
+    int32_t proj;   // rebinned projection index of the view, i + k * (n_proj_turn / 2)
 };
 
 // Host-computed rebinning maps of one block (FP64 geometry rounded to FP32).

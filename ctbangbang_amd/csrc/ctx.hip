@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -50,7 +51,7 @@ struct ctbb_ctx {
     hipStream_t stream = nullptr;
     ctbb_options opt{};
     DevBuf raw_in, raw_af, rebin_tmp, rebin_out, image_block, image_stack, af_work;
-    DevBuf vtab, maps, taps, fin_tab;
+    DevBuf vtab, maps, taps, spec, fin_tab;
     // cached host-side keys
     std::string taps_key, maps_key;
     int map_kind = 0, map_sets = 0;
@@ -103,6 +104,47 @@ int check_geometry(const recon_metadata *mr) {
 }
 
 // Filter taps of the prm's ReconKernel, cached per (install dir, channels, kernel).
+// In-place radix-2 DFT (forward sign) of a power-of-two length, FP64.
+void dft_pow2(std::vector<std::complex<double>> &x) {
+    const size_t n = x.size();
+    for (size_t i = 1, j = 0; i < n; i++) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(x[i], x[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const double ang = -2.0 * M_PI / static_cast<double>(len);
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; k++) {
+                const std::complex<double> w(std::cos(ang * k), std::sin(ang * k));
+                const std::complex<double> u = x[i + k], v = x[i + k + len / 2] * w;
+                x[i + k] = u + v;
+                x[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+// FFT32 form of the filter: H = DFT_4096(taps zero-padded) / 4096 and the
+// twiddles e^(-2 pi i t / 4096), evaluated in FP64 and rounded once, into
+// ctx->spec (only when 2n <= 4096; wider filters use the FP32 product).
+int make_spec(ctbb_ctx *ctx, const float *taps, size_t count) {
+    constexpr size_t L = 4096;
+    if (count > L) return CTBB_OK;
+    std::vector<std::complex<double>> h(L, 0.0);
+    for (size_t m = 0; m < count; m++) h[m] = taps[m];
+    dft_pow2(h);
+    std::vector<float> sp(4 * L);
+    for (size_t k = 0; k < L; k++) {
+        sp[2 * k] = static_cast<float>(h[k].real() / L);
+        sp[2 * k + 1] = static_cast<float>(h[k].imag() / L);
+        const double a = -2.0 * M_PI * static_cast<double>(k) / L;
+        sp[2 * L + 2 * k] = static_cast<float>(std::cos(a));
+        sp[2 * L + 2 * k + 1] = static_cast<float>(std::sin(a));
+    }
+    return upload(ctx, ctx->spec, sp.data(), sp.size() * sizeof(float));
+}
+
 int ensure_taps(ctbb_ctx *ctx, const recon_metadata *mr) {
     std::string key = std::string(mr->install_dir) + "|" + std::to_string(mr->cg.n_channels) + "|" +
                       std::to_string(mr->rp.recon_kernel);
@@ -111,6 +153,8 @@ int ensure_taps(ctbb_ctx *ctx, const recon_metadata *mr) {
     int st = ctbb_load_filter(mr, t.data());
     if (st) return st;
     st = upload(ctx, ctx->taps, t.data(), t.size() * sizeof(float));
+    if (st) return st;
+    st = make_spec(ctx, t.data(), t.size());
     if (st) return st;
     ctx->taps_key = key;
     return CTBB_OK;
@@ -170,6 +214,7 @@ int do_filter(ctbb_ctx *ctx, const float *d_taps, int n, size_t rows, const floa
     f.in = d_in;
     f.out = d_out;
     f.taps = d_taps;
+    f.spec = ctx->spec.p ? ctx->spec.as<float2>() : nullptr;
     f.rows = rows;
     f.n = n;
     f.mode = ctx->opt.filter_mode;
@@ -375,7 +420,7 @@ int ctbb_ctx_create(int device, ctbb_ctx **out) {
         delete c;
         return CTBB_ERR_HIP;
     }
-    c->opt.filter_mode = CTBB_FILTER_F32;
+    c->opt.filter_mode = CTBB_FILTER_FFT32;
     c->opt.proj_dtype = CTBB_PROJ_F32;
     c->opt.mu_water = 0.0f;
     c->opt.bp_tile = 0;
@@ -390,7 +435,7 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->rebin_out, &c->image_block, &c->image_stack,
-                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->fin_tab})
+                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab})
         b->release();
     c->reset_events();
     for (auto &e : c->free_events) {
@@ -409,7 +454,7 @@ int ctbb_ctx_get_options(const ctbb_ctx *c, ctbb_options *o) {
 
 int ctbb_ctx_set_options(ctbb_ctx *c, const ctbb_options *o) {
     if (!c || !o) return CTBB_ERR_ARG;
-    if (o->filter_mode < CTBB_FILTER_F32 || o->filter_mode > CTBB_FILTER_F16X3) return CTBB_ERR_ARG;
+    if (o->filter_mode < CTBB_FILTER_F32 || o->filter_mode > CTBB_FILTER_FFT32) return CTBB_ERR_ARG;
     if (o->proj_dtype != CTBB_PROJ_F32 && o->proj_dtype != CTBB_PROJ_BF16) return CTBB_ERR_ARG;
     c->opt = *o;
     return CTBB_OK;
@@ -461,6 +506,16 @@ int ctbb_filter_rows_dev(ctbb_ctx *c, const float *d_taps, int n, size_t n_rows_
     if (!c || !d_taps || !d_in || !d_out || n <= 0) return CTBB_ERR_ARG;
     hipSetDevice(c->device);
     c->reset_events();
+    if (c->opt.filter_mode == CTBB_FILTER_FFT32) {
+        // the spectrum of these taps (the cached one belongs to a prm's filter)
+        std::vector<float> t(2 * static_cast<size_t>(n));
+        if (hipStreamSynchronize(c->stream) != hipSuccess ||
+            hipMemcpy(t.data(), d_taps, t.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            return CTBB_ERR_HIP;
+        int st = make_spec(c, t.data(), t.size());
+        if (st) return st;
+        c->taps_key.clear();
+    }
     return do_filter(c, d_taps, n, n_rows_total, d_in, d_out, c->opt.proj_dtype == CTBB_PROJ_BF16);
 }
 

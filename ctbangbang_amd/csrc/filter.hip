@@ -14,6 +14,8 @@
 //           operands: X*T ~= Xh*Th + Xh*Tl + Xl*Th, FP32 accumulation
 //   F16X3   the same with float16 splits; taps are pre-scaled by 2^10 (and
 //           the result by 2^-10) to keep the low parts out of the subnormals
+//   FFT32   FP32 4096-point FFT convolution (the form of the reference's CPU
+//           path), for n <= 2048; wider rows fall back to F32
 // Tile: 64 rows x 64 columns per workgroup (4 waves, 2x2 of 32x32 per wave),
 // K chunks of 32 staged in LDS.  Requires n % 64 == 0 (1472 and 1344 for the
 // 736- and 672-channel scanners); other widths use a VALU kernel.
@@ -282,11 +284,160 @@ __global__ __launch_bounds__(256) void conv_valu_kernel(FilterArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 FFT form of the same convolution (CTBB_FILTER_FFT32).
+// Embedding: with L = 4096 >= 2n, h[m] = taps[m] for m < 2n and 0 above, the
+// circular convolution y = x (*) h of the zero-padded row gives
+// y[n + l] = sum_k x[k] taps[n + l - k] for l in [0, n) without wrap-around
+// (n + l - k lies in [1, 2n - 1]).  Two rows a, b travel as one complex
+// signal a + i b: h is real, so y = (a (*) h) + i (b (*) h).
+// The host supplies H = DFT_L(h) / L and the twiddles e^(-2 pi i t / L),
+// both evaluated in FP64 and rounded once.
+// Transform: Stockham radix-16 (three passes of 256 threads x 16 points,
+// the passes in between through LDS with one pad entry per 16 to spread
+// the banks); the inverse is conj(DFT(conj(Y))).
+constexpr int FFT_L = 4096;
+constexpr int FFT_LDS = FFT_L + FFT_L / 16;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// in-place 4-point DFT, forward sign: X1 = a0 - i a1 - a2 + i a3
+__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
+    const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = make_float2(d02.x + d13.y, d02.y - d13.x);
+    a3 = make_float2(d02.x - d13.y, d02.y + d13.x);
+}
+
+// 16-point DFT in natural order, as 4 x 4: inner DFTs over r2 (r = r1 + 4 r2),
+// twiddles W16^(r1 k2), outer DFTs over r1; X[k2 + 4 k1] ends in slot k1 + 4 k2.
+__device__ __forceinline__ void fft16(float2 (&v)[16]) {
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, H2 = 0.70710678118654757f;
+#pragma unroll
+    for (int r1 = 0; r1 < 4; r1++) dft4(v[r1], v[r1 + 4], v[r1 + 8], v[r1 + 12]);
+    // W16^m = (cos(pi m / 8), -sin(pi m / 8))
+    v[5] = cmul(v[5], make_float2(C1, -S1));     // r1 1, k2 1: m 1
+    v[9] = cmul(v[9], make_float2(H2, -H2));     // r1 1, k2 2: m 2
+    v[13] = cmul(v[13], make_float2(S1, -C1));   // r1 1, k2 3: m 3
+    v[6] = cmul(v[6], make_float2(H2, -H2));     // r1 2, k2 1: m 2
+    v[10] = make_float2(v[10].y, -v[10].x);      // r1 2, k2 2: m 4 (-i)
+    v[14] = cmul(v[14], make_float2(-H2, -H2));  // r1 2, k2 3: m 6
+    v[7] = cmul(v[7], make_float2(S1, -C1));     // r1 3, k2 1: m 3
+    v[11] = cmul(v[11], make_float2(-H2, -H2));  // r1 3, k2 2: m 6
+    v[15] = cmul(v[15], make_float2(-C1, S1));   // r1 3, k2 3: m 9
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) dft4(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3]);
+    // slot k1 + 4 k2 holds X[k2 + 4 k1]: transpose to natural order
+    float2 t;
+    t = v[1]; v[1] = v[4]; v[4] = t;
+    t = v[2]; v[2] = v[8]; v[8] = t;
+    t = v[3]; v[3] = v[12]; v[12] = t;
+    t = v[6]; v[6] = v[9]; v[9] = t;
+    t = v[7]; v[7] = v[13]; v[13] = t;
+    t = v[11]; v[11] = v[14]; v[14] = t;
+}
+
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
+
+// Forward 4096-point DFT.  On entry v[r] = x[j + 256 r]; on exit
+// v[r] = X[j + 256 r].  tw: e^(-2 pi i t / 4096), t in [0, 4096).
+__device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, const float2 *__restrict__ tw) {
+    fft16(v);                                           // pass Ns = 1: no twiddles
+    __syncthreads();                                    // earlier readers of sh are done
+#pragma unroll
+    for (int r = 0; r < 16; r++) sh[fpad(16 * j + r)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 256 * r)];
+#pragma unroll
+    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], tw[(16 * (j & 15) * r) & (FFT_L - 1)]);
+    fft16(v);                                           // pass Ns = 16
+    __syncthreads();
+    const int base = (j >> 4) * 256 + (j & 15);
+#pragma unroll
+    for (int r = 0; r < 16; r++) sh[fpad(base + 16 * r)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 256 * r)];
+#pragma unroll
+    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], tw[j * r]);
+    fft16(v);                                           // pass Ns = 256
+}
+
+// One workgroup per pair of rows; spec = [H (L)] followed by [twiddles (L)]
+// as float2.
+__global__ __launch_bounds__(256) void fft_filter_kernel(FilterArgs a) {
+    __shared__ float2 sh[FFT_LDS];
+    const int j = threadIdx.x;
+    const int n = a.n;
+    const float2 *__restrict__ H = a.spec;
+    const float2 *__restrict__ tw = a.spec + FFT_L;
+    const size_t ra = 2 * static_cast<size_t>(blockIdx.x);
+    const bool has_b = ra + 1 < a.rows;
+    const float *xa = a.in + ra * n;
+    const float *xb = xa + n;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = j + 256 * r;
+        v[r] = make_float2(m < n ? xa[m] : 0.0f, (m < n && has_b) ? xb[m] : 0.0f);
+    }
+    fft4096(v, sh, j, tw);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const float2 y = cmul(v[r], H[j + 256 * r]);
+        v[r] = make_float2(y.x, -y.y);
+    }
+    fft4096(v, sh, j, tw);
+    // y = conj(v); keep y[n .. 2n)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = j + 256 * r;
+        if (m >= n && m < 2 * n) {
+            const size_t o = ra * n + (m - n);
+            if (a.out_bf16) {
+                uint16_t *ob = reinterpret_cast<uint16_t *>(a.out);
+                ob[o] = f32_to_bf16_rne(v[r].x);
+                if (has_b) ob[o + n] = f32_to_bf16_rne(-v[r].y);
+            } else {
+                float *of = reinterpret_cast<float *>(a.out);
+                of[o] = v[r].x;
+                if (has_b) of[o + n] = -v[r].y;
+            }
+        }
+    }
+}
+
 }  // namespace
 
 int launch_filter(const FilterArgs &a, hipStream_t stream) {
     if (!a.in || !a.out || !a.taps || a.n <= 0) return CTBB_ERR_ARG;
     if (a.rows == 0) return CTBB_OK;
+    if (a.mode == CTBB_FILTER_FFT32 && 2 * a.n <= FFT_L) {
+        // each workgroup reads its two rows completely before writing them,
+        // so in == out is allowed here; grid.x is limited to 2^31 - 1
+        if (!a.spec) return CTBB_ERR_ARG;
+        const size_t n_pairs = (a.rows + 1) / 2;
+        const size_t max_pairs = 0x7fffffffu;
+        for (size_t p0 = 0; p0 < n_pairs; p0 += max_pairs) {
+            FilterArgs s = a;
+            const size_t np = n_pairs - p0 < max_pairs ? n_pairs - p0 : max_pairs;
+            s.in = a.in + 2 * p0 * a.n;
+            if (a.out_bf16) s.out = reinterpret_cast<uint16_t *>(a.out) + 2 * p0 * a.n;
+            else s.out = reinterpret_cast<float *>(a.out) + 2 * p0 * a.n;
+            s.rows = a.rows - 2 * p0 < 2 * np ? a.rows - 2 * p0 : 2 * np;
+            hipLaunchKernelGGL(fft_filter_kernel, dim3(static_cast<unsigned>(np)), dim3(256), 0, stream, s);
+            if (hipGetLastError() != hipSuccess) return CTBB_ERR_HIP;
+        }
+        return CTBB_OK;
+    }
+    // FFT32 with 2n > 4096 uses the FP32 Toeplitz product below
     if (a.in == a.out && (a.out_bf16 || a.n % 64 == 0)) return CTBB_ERR_ARG;  // tiled kernel reads rows after writing others
     if (a.n % 64 != 0) {
         if (a.rows > 0x7fffffffu) return CTBB_ERR_SHAPE;
@@ -296,7 +447,7 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
         return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
     }
     const size_t gy = (a.rows + BM - 1) / BM;
-    const size_t lds = a.mode == CTBB_FILTER_F32
+    const size_t lds = (a.mode == CTBB_FILTER_F32 || a.mode == CTBB_FILTER_FFT32)
                            ? BM * A_STRIDE * sizeof(float) + 2 * a.n * sizeof(float)
                            : (2 * BM * (BK + 8) + 8 * (static_cast<size_t>(a.n) + 72)) * sizeof(uint16_t);
     if (lds > 64 * 1024) return CTBB_ERR_SHAPE;
@@ -311,7 +462,8 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
         s.rows = a.rows - y0 * BM < ny * BM ? a.rows - y0 * BM : ny * BM;
         dim3 grid(a.n / BN, static_cast<unsigned>(ny));
         switch (a.mode) {
-            case CTBB_FILTER_F32: hipLaunchKernelGGL(toeplitz_f32_kernel, grid, dim3(256), lds, stream, s); break;
+            case CTBB_FILTER_F32:
+            case CTBB_FILTER_FFT32: hipLaunchKernelGGL(toeplitz_f32_kernel, grid, dim3(256), lds, stream, s); break;
             case CTBB_FILTER_BF16X3: hipLaunchKernelGGL(toeplitz_split_kernel<CTBB_FILTER_BF16X3>, grid, dim3(256), lds, stream, s); break;
             case CTBB_FILTER_F16X3: hipLaunchKernelGGL(toeplitz_split_kernel<CTBB_FILTER_F16X3>, grid, dim3(256), lds, stream, s); break;
             default: return CTBB_ERR_ARG;

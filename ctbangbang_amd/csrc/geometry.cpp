@@ -75,8 +75,8 @@ void bp_view_table(const recon_metadata &mr, std::vector<BpView> &tab) {
     const double ts = static_cast<double>(bp_tube_start(mr));
     const double zr = cg.z_rot;
     tab.resize(static_cast<size_t>(half) * n_ht);
-    for (int k = 0; k < n_ht; k++)
-        for (int i = 0; i < half; i++) {
+    for (int i = 0; i < half; i++)
+        for (int k = 0; k < n_ht; k++) {
             // theta = tube_start + (2 pi'/n_proj_turn) i + k pi'  (backproject.cuh:75)
             double dtheta = (2.0 * pi / static_cast<double>(cg.n_proj_turn)) * i + k * pi;
             double th = ts + dtheta;
@@ -85,8 +85,8 @@ void bp_view_table(const recon_metadata &mr, std::vector<BpView> &tab) {
             v.cos_t = static_cast<float>(std::cos(th));
             // z - ray_pos = zi*coll + z_rot/2 + z_rot*(tube_start - theta)/(2 pi') + z_rot*asin(phat/r_f)/(2 pi')
             v.c_v = static_cast<float>(zr / 2.0 - zr * dtheta / (2.0 * pi));
-            v.pad = 0.0f;
-            tab[static_cast<size_t>(k) * half + i] = v;
+            v.proj = i + k * half;
+            tab[static_cast<size_t>(i) * n_ht + k] = v;
         }
 }
 

@@ -51,6 +51,7 @@ struct FilterArgs {
     const float *in;           // [rows][n]
     void *out;                 // [rows][n] float32 or bfloat16
     const float *taps;         // [2n]
+    const float2 *spec;        // FFT32: [H (4096)][twiddles (4096)]
     size_t rows;
     int n;
     int mode;                  // ctbb_filter_mode

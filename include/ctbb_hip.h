@@ -123,7 +123,8 @@ typedef struct ctbb_ctx ctbb_ctx;
 typedef enum {
     CTBB_FILTER_F32 = 0,        /* FP32 Toeplitz product, FP32 accumulation */
     CTBB_FILTER_BF16X3 = 1,     /* BF16 hi+lo split operands on MFMA, FP32 accumulation */
-    CTBB_FILTER_F16X3 = 2       /* FP16 hi+lo split operands on MFMA, FP32 accumulation */
+    CTBB_FILTER_F16X3 = 2,      /* FP16 hi+lo split operands on MFMA, FP32 accumulation */
+    CTBB_FILTER_FFT32 = 3       /* FP32 4096-point FFT convolution (rows up to 2048 channels; wider: F32) */
 } ctbb_filter_mode;
 
 typedef enum {
@@ -132,7 +133,7 @@ typedef enum {
 } ctbb_proj_dtype;
 
 typedef struct ctbb_options_s {
-    int filter_mode;            /* ctbb_filter_mode */
+    int filter_mode;            /* ctbb_filter_mode; default CTBB_FILTER_FFT32 */
     int proj_dtype;             /* ctbb_proj_dtype */
     float mu_water;             /* >0: convert the final stack to HU */
     int bp_tile;                /* backprojection slices per thread: 16 or 32 (0 = default 32) */

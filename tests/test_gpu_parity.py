@@ -55,7 +55,8 @@ def test_synth_host_matches_device(ctx, tmp_path, mini_dir):
     cb.free_metadata(mr)
 
 
-@pytest.mark.parametrize("mode,tol", [(cb.FILTER_F32, 2e-6), (cb.FILTER_BF16X3, 3e-5), (cb.FILTER_F16X3, 2e-6)])
+@pytest.mark.parametrize("mode,tol", [(cb.FILTER_F32, 2e-6), (cb.FILTER_BF16X3, 3e-5), (cb.FILTER_F16X3, 2e-6),
+                                      (cb.FILTER_FFT32, 2e-6)])
 @pytest.mark.parametrize("n", [1472, 256, 1840])
 def test_ramp_filter(ctx, mode, tol, n):
     import oracle
@@ -75,7 +76,23 @@ def test_ramp_filter(ctx, mode, tol, n):
     ctx.set_options(filter_mode=cb.FILTER_F32)
 
 
-@pytest.mark.parametrize("mode", [cb.FILTER_F32, cb.FILTER_BF16X3])
+def test_ramp_filter_fft_in_place(ctx):
+    """FFT32 form filtering a buffer in place (each workgroup reads its rows
+    before writing them): same result as out of place."""
+    n, rows = 1472, 101
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((rows, n)).astype(np.float32)
+    taps = cb.ramp_filter(n, 0.35, 1.0, 1.0)
+    ctx.set_options(filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
+    a = dev(x)
+    out = torch.empty(rows * n, dtype=torch.float32, device="cuda")
+    ctx.filter_rows_dev(dev(taps), n, rows, a, out)
+    ctx.filter_rows_dev(dev(taps), n, rows, a, a)
+    ctx.set_options(filter_mode=cb.FILTER_F32)
+    assert np.array_equal(host(a).ravel(), host(out))
+
+
+@pytest.mark.parametrize("mode", [cb.FILTER_F32, cb.FILTER_BF16X3, cb.FILTER_FFT32])
 def test_ramp_filter_bf16_output(ctx, mode):
     """bfloat16 storage of the filtered rows: round-to-nearest-even of the
     FP32-accumulated result (<= 2^-8 relative per sample, plus the filter's
@@ -282,7 +299,8 @@ def test_end_to_end_hu(ctx, tmp_path, mini_dir, ffs, table_dir):
 
 
 @pytest.mark.parametrize("mode,proj", [(cb.FILTER_BF16X3, cb.PROJ_F32), (cb.FILTER_F16X3, cb.PROJ_F32),
-                                       (cb.FILTER_BF16X3, cb.PROJ_BF16)])
+                                       (cb.FILTER_BF16X3, cb.PROJ_BF16), (cb.FILTER_FFT32, cb.PROJ_F32),
+                                       (cb.FILTER_FFT32, cb.PROJ_BF16)])
 def test_end_to_end_precision_form_hu(ctx, tmp_path, mini_dir, mode, proj):
     """Precision form (split MFMA filter, optionally BF16 projections with FP32
     accumulation) vs the FP64 oracle: the north-star bar is <= 1 HU RMS."""
