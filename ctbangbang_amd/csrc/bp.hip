@@ -232,7 +232,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
         float asn = asin_small(xr);
         if (__any(fabsf(xr) > 0.45f)) asn = fabsf(xr) > 0.45f ? asinf(xr) : asn;
         const float dz0 = vw.z + a.k_asin * asn + static_cast<float>(z0) * a.coll;
-        const float lhat = sqrtf(a.rf2 - phat * phat) - x * vw.y - y * vw.x;
+        // hardware square root (1 ulp): no denormal scaling or correction steps
+        const float lhat = __builtin_amdgcn_sqrtf(a.rf2 - phat * phat) - x * vw.y - y * vw.x;
         const float g = a.neg_td * __builtin_amdgcn_rcpf(lhat * a.tanc);
         const float A = g * a.coll;
         const float B = g * dz0;
@@ -240,34 +241,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
         const unsigned colb = static_cast<unsigned>((t & 1) * n_stage) + static_cast<unsigned>(c0) +
                               static_cast<unsigned>(c0) / CPT;
 
-        // Slices in chunks of 4: one wave vote per chunk skips chunks no lane
-        // sees; the 4 slices of a visible chunk run without branches (W = 0 for
-        // lanes that do not see a slice, row coordinate clamped into the
-        // window), so their LDS reads are in flight together.  The taper is
-        // evaluated only for chunks where some lane has 0.6 <= |q| < 1.
+        // Slices in chunks of 4.  q is linear in z, so the chunk's q range is
+        // spanned by its end slices: one wave vote skips chunks no lane sees,
+        // and a second finds chunks where every lane stays inside |q| < 0.6
+        // (W = 1 throughout, the common case).  Otherwise W is evaluated per
+        // slice, the taper only in lanes with 0.6 <= |q| < 1.  The 4 slices
+        // of a visible chunk then run without branches (W = 0 for lanes that
+        // do not see a slice, row coordinate clamped into the window), so
+        // their LDS reads are in flight together.
         const float two_hr = 2.0f * a.hr;
 #pragma unroll
         for (int zc = 0; zc < NZ; zc += 4) {
+            const float qa = fmaf(A, static_cast<float>(zc), B);
+            const float qb = fmaf(A, static_cast<float>(zc + 3), B);
+            const float lo = fminf(qa, qb), hi = fmaxf(qa, qb);
+            if (__builtin_amdgcn_ballot_w64(lo < 1.0f && hi > -1.0f) == 0) continue;
             float q[4], w[4];
-            unsigned long long mv = 0, mt = 0;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                q[c] = fmaf(A, static_cast<float>(zc + c), B);
-                const float aq = fabsf(q[c]);
-                mv |= __builtin_amdgcn_ballot_w64(aq < 1.0f);
-                mt |= __builtin_amdgcn_ballot_w64(aq < 1.0f && aq >= 0.6f);
-            }
-            if (mv == 0) continue;
-            if (mt != 0) {
+            for (int c = 0; c < 4; c++) q[c] = fmaf(A, static_cast<float>(zc + c), B);
+            if (__builtin_amdgcn_ballot_w64(lo <= -0.6f || hi >= 0.6f) == 0) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) w[c] = 1.0f;
+            } else {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
                     const float aq = fabsf(q[c]);
-                    const float tw = aq >= 0.6f ? stier_taper(aq) : 1.0f;
-                    w[c] = aq < 1.0f ? tw : 0.0f;
+                    float wc = aq < 1.0f ? 1.0f : 0.0f;
+                    if (aq < 1.0f && aq >= 0.6f) wc = stier_taper(aq);
+                    w[c] = wc;
                 }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; c++) w[c] = fabsf(q[c]) < 1.0f ? 1.0f : 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < 4; c++) {
