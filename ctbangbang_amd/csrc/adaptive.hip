@@ -25,19 +25,31 @@ namespace ctbb {
 
 namespace {
 
+__device__ __forceinline__ float wave_max(float v) {
+    for (int s = 32; s > 0; s >>= 1) v = fmaxf(v, __shfl_xor(v, s, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int s = 32; s > 0; s >>= 1) v = fminf(v, __shfl_xor(v, s, 64));
+    return v;
+}
+
+// View maximum, one workgroup per view (max is order-free, so any reduction
+// order gives the reference's value).  frame % 4 == 0 is checked on the host.
 __global__ __launch_bounds__(256) void af_sup(const float *raw, float *sup, size_t frame) {
-    __shared__ float red[256];
+    __shared__ float red[4];
     const size_t p = blockIdx.x;
-    const float *f = raw + p * frame;
+    const float4 *f = reinterpret_cast<const float4 *>(raw + p * frame);
     float m = 0.0f;   // reference starts from 0.0f (preprocessing.cuh:75)
-    for (size_t i = threadIdx.x; i < frame; i += 256) m = fmaxf(m, f[i]);
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
+    for (size_t i = threadIdx.x; i < frame / 4; i += 256) {
+        const float4 v = f[i];
+        m = fmaxf(fmaxf(m, fmaxf(v.x, v.y)), fmaxf(v.z, v.w));
     }
-    if (threadIdx.x == 0) sup[p] = red[0];
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) sup[p] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
 // reference preprocessing.cuh:87-113
@@ -56,20 +68,25 @@ __global__ void af_smooth(const float *sup, float *sm, int n_proj, int n_avg) {
     sm[p] = acc;
 }
 
-// reference preprocessing.cuh:115-163
-__global__ void af_ecc(const float *sm, float *mx, float *mn, float *ecc, int n_proj, int n_range) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// reference preprocessing.cuh:115-163; one wavefront per view (min and max
+// are order-free)
+__global__ __launch_bounds__(256) void af_ecc(const float *sm, float *mx, float *mn, float *ecc, int n_proj,
+                                              int n_range) {
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n_proj) return;
     const int off = n_range / 2;
     const int max_idx = n_proj - 1;
     float hi = 0.0f, lo = 10000.0f;
-    for (int i = 0; i < n_range; i++) {
+    for (int i = threadIdx.x & 63; i < n_range; i += 64) {
         int idx = p - off + i;
         idx = idx < 0 ? 0 : (idx > max_idx ? max_idx : idx);
         const float v = sm[idx];
-        if (v > hi) hi = v;
-        if (v < lo) lo = v;
+        hi = fmaxf(hi, v);
+        lo = fminf(lo, v);
     }
+    hi = wave_max(hi);
+    lo = wave_min(lo);
+    if ((threadIdx.x & 63) != 0) return;
     mx[p] = hi;
     mn[p] = lo;
     float e = 1 - lo / hi;
@@ -163,6 +180,7 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     if (!a.raw || !a.out || !a.work || a.raw == a.out) return CTBB_ERR_ARG;
     if (a.n_proj <= 0 || a.n_rows_raw <= 0 || a.nch <= 0 || a.n_proj_ffs <= 0) return CTBB_ERR_SHAPE;
     if (a.n_proj > 65535 || a.n_rows_raw > 65535) return CTBB_ERR_SHAPE;  // grid.z / grid.y limits
+    if ((static_cast<size_t>(a.n_rows_raw) * a.nch) % 4 != 0) return CTBB_ERR_SHAPE;  // float4 frames
     const size_t np = static_cast<size_t>(a.n_proj);
     float *sup = a.work, *sm = sup + np, *mx = sm + np, *mn = mx + np, *ecc = mn + np, *thr = ecc + np;
     const size_t frame = static_cast<size_t>(a.n_rows_raw) * a.nch;
@@ -171,7 +189,7 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     const unsigned g1 = static_cast<unsigned>((a.n_proj + 255) / 256);
     hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
     hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
-    hipLaunchKernelGGL(af_ecc, dim3(g1), dim3(256), 0, stream, sm, mx, mn, ecc, a.n_proj, n_range);
+    hipLaunchKernelGGL(af_ecc, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, a.n_proj, n_range);
     hipLaunchKernelGGL(af_thresh, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
                        n_range, a.n_proj_ffs, a.strength);
     hipLaunchKernelGGL(af_apply, dim3((a.nch + 255) / 256, a.n_rows_raw, a.n_proj), dim3(256), 0, stream, a.raw,

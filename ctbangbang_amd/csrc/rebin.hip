@@ -25,7 +25,7 @@ namespace ctbb {
 
 namespace {
 
-constexpr int kViewsPerThread = 4;
+constexpr int kViewsPerThread = 8;
 
 // Clamp a fractional index into [0, n-1] and split it into the two taps and
 // the weight of the upper tap (texture clamp addressing).
@@ -42,6 +42,9 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
     const int r = blockIdx.y;
     if (c >= a.nos) return;
     const size_t frame = static_cast<size_t>(a.n_rows_raw) * a.nch;
+    const float *__restrict__ raw = a.raw;
+    const float *__restrict__ aoff = a.aoff;
+    float *__restrict__ outp = a.out;
 
     int set, j;
     if (a.kind == 0 || a.kind == 2) {
@@ -55,6 +58,18 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
     }
     const float by = a.bidx[set * a.nos + c];
 
+    // per-thread constants of the output channel (independent of the view)
+    int c0 = 0, c1 = 0, m0 = 0, m1 = 0;
+    float fc = 0.0f, fm = 0.0f, ao0, ao1 = 0.0f;
+    if (a.kind <= 1) {
+        taps(by, a.nch, c0, c1, fc);
+        ao0 = aoff[set * a.nos + c];
+    } else {
+        taps(by, a.nos, m0, m1, fm);
+        ao0 = aoff[set * a.nos + m0];
+        ao1 = aoff[set * a.nos + m1];
+    }
+
     for (int vv = 0; vv < kViewsPerThread; vv++) {
         const int v = blockIdx.z * kViewsPerThread + vv;
         if (v >= a.n_proj_out) break;
@@ -63,11 +78,10 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
         if (a.kind <= 1) {
             // single 2D bilinear fetch in (channel, sheet projection)
             const int fs = a.kind == 1 ? set : 0;
-            int c0, c1, p0, p1;
-            float fc, fa;
-            taps(by, a.nch, c0, c1, fc);
-            taps(proj - a.aoff[set * a.nos + c], a.n_proj, p0, p1, fa);
-            const float *row = a.raw + static_cast<size_t>(j) * a.nch;
+            int p0, p1;
+            float fa;
+            taps(proj - ao0, a.n_proj, p0, p1, fa);
+            const float *row = raw + static_cast<size_t>(j) * a.nch;
             const float *s0 = row + (static_cast<size_t>(p0) * a.n_ffs + fs) * frame;
             const float *s1 = row + (static_cast<size_t>(p1) * a.n_ffs + fs) * frame;
             const float t0 = fmaf(fc, s0[c1] - s0[c0], s0[c0]);
@@ -76,9 +90,6 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
         } else {
             // two-pass modes: radial tap pair over the interleaved azimuthal
             // rebin of the focal-spot pair `set`
-            int m0, m1;
-            float fm;
-            taps(by, a.nos, m0, m1, fm);
             float t[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -87,15 +98,15 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
                 const int ch = m >> 1;
                 int p0, p1;
                 float fa;
-                taps(proj - a.aoff[set * a.nos + m], a.n_proj, p0, p1, fa);
-                const float *row = a.raw + static_cast<size_t>(j) * a.nch + ch;
+                taps(proj - (h ? ao1 : ao0), a.n_proj, p0, p1, fa);
+                const float *row = raw + static_cast<size_t>(j) * a.nch + ch;
                 const float u0 = row[(static_cast<size_t>(p0) * a.n_ffs + fs) * frame];
                 const float u1 = row[(static_cast<size_t>(p1) * a.n_ffs + fs) * frame];
                 t[h] = fmaf(fa, u1 - u0, u0);
             }
             out = fmaf(fm, t[1] - t[0], t[0]);
         }
-        a.out[(static_cast<size_t>(v) * a.n_rows + r) * a.nos + c] = out;
+        outp[(static_cast<size_t>(v) * a.n_rows + r) * a.nos + c] = out;
     }
 }
 
