@@ -95,6 +95,7 @@ _SIGS = {
     "ctbb_backproject_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_finalize_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_reconstruct_dev": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP, _VP]),
+    "ctbb_reconstruct_host": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP]),
     "ctbb_synth_scan": (C.c_int, [_MR, _P(PhantomParams), C.c_long, C.c_long, _P(C.c_float)]),
     "ctbb_synth_scan_dev": (C.c_int, [_VP, _MR, _P(PhantomParams), C.c_long, C.c_long, _VP]),
 }
@@ -296,6 +297,12 @@ class Context:
     def reconstruct_dev(self, mr, d_raw, n_raw_readings, d_final, d_image=None):
         _check(lib().ctbb_reconstruct_dev(self._h, C.byref(mr), _ptr(d_raw), int(n_raw_readings), _ptr(d_image),
                                           _ptr(d_final)), "reconstruct_dev")
+
+    def reconstruct_host(self, mr, h_raw, n_raw_readings, h_final):
+        """Whole reconstruction from a host-resident scan (page-locked memory
+        overlaps the uploads with the compute); h_final is host memory."""
+        _check(lib().ctbb_reconstruct_host(self._h, C.byref(mr), _ptr(h_raw), int(n_raw_readings), _ptr(h_final)),
+               "reconstruct_host")
 
     def synth_scan_dev(self, mr, pp, first, count, d_out):
         _check(lib().ctbb_synth_scan_dev(self._h, C.byref(mr), C.byref(pp), int(first), int(count), _ptr(d_out)),

@@ -5,6 +5,7 @@
 // recorded around every launch so callers can read per-kernel device time.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -52,6 +53,11 @@ struct ctbb_ctx {
     ctbb_options opt{};
     DevBuf raw_in, raw_af, rebin_tmp, rebin_out, image_block, image_stack, af_work;
     DevBuf vtab, maps, taps, spec, fin_tab;
+    // host-streaming reconstruction: two raw pull buffers filled on a copy
+    // stream (default priority) while the compute stream works on the other
+    DevBuf pull[2];
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
     // cached host-side keys
     std::string taps_key, maps_key;
     int map_kind = 0, map_sets = 0;
@@ -91,6 +97,21 @@ int upload(ctbb_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
     if (hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return CTBB_ERR_HIP;
     // the host staging buffer may be reused as soon as this returns
     return hip_ok(hipStreamSynchronize(ctx->stream));
+}
+
+// Zero fill on a stream (a kernel rather than hipMemsetAsync, so the
+// sequence stays valid inside a captured graph).
+__global__ void fill_zero_kernel(float *p, size_t n) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        p[i] = 0.0f;
+}
+
+int fill_zero(float *p, size_t n, hipStream_t stream) {
+    if (n == 0) return CTBB_OK;
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 16384));
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid), dim3(256), 0, stream, p, n);
+    return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 
 int check_geometry(const recon_metadata *mr) {
@@ -401,6 +422,37 @@ int backproject_dev(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin
     return do_backproject(ctx, mr, d_rebin, ctx->vtab.as<BpView>(), d_img);
 }
 
+// Adaptive filter -> rebin -> ramp filter -> backprojection of block b of
+// the plan (mr->ri.cb set to it), raw pull at `win` on the device.
+int process_block(ctbb_ctx *c, recon_metadata *mr, int b, const float *win, const BpView *d_vtab,
+                  float *d_image) {
+    const ct_geom &cg = mr->cg;
+    const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
+    const size_t frame = cg.n_rows_raw * cg.n_channels;
+    const size_t n = mr->ri.n_proj_pull;
+    ctbb_sizes sz;
+    ctbb_block_sizes(mr, &sz);
+    int st;
+    if (c->opt.adaptive_filter) {
+        st = c->raw_af.ensure(n * frame * sizeof(float));
+        if (st) return st;
+        st = do_adaptive(c, mr, win, c->raw_af.as<float>());
+        if (st) return st;
+        win = c->raw_af.as<float>();
+    }
+    st = c->rebin_tmp.ensure(sz.rebin * sizeof(float));
+    if (st) return st;
+    st = do_rebin(c, mr, win, c->rebin_tmp.as<float>());
+    if (st) return st;
+    st = c->rebin_out.ensure(sz.rebin * proj_elem_bytes(c));
+    if (st) return st;
+    st = do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), sz.n_proj_out * cg.n_rows,
+                   c->rebin_tmp.as<float>(), c->rebin_out.p, c->opt.proj_dtype == CTBB_PROJ_BF16);
+    if (st) return st;
+    return do_backproject(c, mr, c->rebin_out.p, d_vtab,
+                          d_image + static_cast<size_t>(b) * mr->ri.n_slices_block * nxy);
+}
+
 }  // namespace
 
 extern "C" {
@@ -434,9 +486,15 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
     for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->rebin_out, &c->image_block, &c->image_stack,
-                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab})
+                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1]})
         b->release();
+    for (int i = 0; i < 2; i++) {
+        if (c->ev_ready[i]) hipEventDestroy(c->ev_ready[i]);
+        if (c->ev_free[i]) hipEventDestroy(c->ev_free[i]);
+    }
+    if (c->copy_stream) hipStreamDestroy(c->copy_stream);
     c->reset_events();
     for (auto &e : c->free_events) {
         hipEventDestroy(e.a);
@@ -588,8 +646,8 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
             st = c->raw_in.ensure(static_cast<size_t>(n) * frame * sizeof(float));
             if (st) return st;
             float *dst = c->raw_in.as<float>();
-            if (hipMemsetAsync(dst, 0, static_cast<size_t>(n) * frame * sizeof(float), c->stream) != hipSuccess)
-                return CTBB_ERR_HIP;
+            st = fill_zero(dst, static_cast<size_t>(n) * frame, c->stream);
+            if (st) return st;
             long lo = start < 0 ? 0 : start, hi = start + n;
             if (hi > static_cast<long>(n_raw_readings)) hi = static_cast<long>(n_raw_readings);
             if (hi > lo &&
@@ -599,27 +657,125 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
                 return CTBB_ERR_HIP;
             win = dst;
         }
-        if (c->opt.adaptive_filter) {
-            st = c->raw_af.ensure(static_cast<size_t>(n) * frame * sizeof(float));
-            if (st) return st;
-            st = do_adaptive(c, mr, win, c->raw_af.as<float>());
-            if (st) return st;
-            win = c->raw_af.as<float>();
-        }
-        st = c->rebin_tmp.ensure(sz.rebin * sizeof(float));
-        if (st) return st;
-        st = do_rebin(c, mr, win, c->rebin_tmp.as<float>());
-        if (st) return st;
-        st = c->rebin_out.ensure(sz.rebin * proj_elem_bytes(c));
-        if (st) return st;
-        st = do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), sz.n_proj_out * cg.n_rows,
-                       c->rebin_tmp.as<float>(), c->rebin_out.p, c->opt.proj_dtype == CTBB_PROJ_BF16);
-        if (st) return st;
-        st = do_backproject(c, mr, c->rebin_out.p, c->vtab.as<BpView>() + off[b],
-                            d_image + static_cast<size_t>(b) * mr->ri.n_slices_block * nxy);
+        st = process_block(c, mr, b, win, c->vtab.as<BpView>() + off[b], d_image);
         if (st) return st;
     }
     return do_finalize(c, mr, d_image, d_final);
+}
+
+int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h_raw, size_t n_raw_readings,
+                          float *h_final) {
+    if (!c || !mr || !h_raw || !h_final || !mr->table_positions || !mr->tube_angles) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    const ct_geom &cg = mr->cg;
+    const int n_blocks = static_cast<int>(mr->ri.n_blocks);
+    if (n_blocks <= 0) return CTBB_ERR_ARG;
+    const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
+    const size_t frame = cg.n_rows_raw * cg.n_channels;
+    int st = c->image_stack.ensure(mr->ri.n_slices_recon * nxy * sizeof(float));
+    if (st) return st;
+    float *d_image = c->image_stack.as<float>();
+    std::vector<BpView> all;
+    std::vector<size_t> off(n_blocks);
+    std::vector<long> start(n_blocks), count(n_blocks);
+    size_t max_pull = 0;
+    for (int b = 0; b < n_blocks; b++) {
+        ctbb::plan_block(mr, b);
+        st = check_geometry(mr);
+        if (st) return st;
+        std::vector<BpView> t;
+        bp_view_table(*mr, t);
+        off[b] = all.size();
+        all.insert(all.end(), t.begin(), t.end());
+        start[b] = static_cast<long>(static_cast<int>(mr->ri.idx_pull_start));
+        count[b] = static_cast<long>(mr->ri.n_proj_pull);
+        max_pull = std::max(max_pull, static_cast<size_t>(count[b]));
+    }
+    st = upload(c, c->vtab, all.data(), all.size() * sizeof(BpView));
+    if (st) return st;
+    st = ensure_taps(c, mr);
+    if (st) return st;
+    st = ensure_maps(c, mr);
+    if (st) return st;
+    if (!c->copy_stream) {
+        if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) return CTBB_ERR_HIP;
+        for (int i = 0; i < 2; i++)
+            if (hipEventCreateWithFlags(&c->ev_ready[i], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_free[i], hipEventDisableTiming) != hipSuccess)
+                return CTBB_ERR_HIP;
+    }
+    for (int i = 0; i < 2; i++) {
+        st = c->pull[i].ensure(max_pull * frame * sizeof(float));
+        if (st) return st;
+    }
+    // the copy stream must not overwrite a buffer before earlier work on the
+    // compute stream (a previous call) is done with it
+    if (hipEventRecord(c->ev_free[0], c->stream) != hipSuccess ||
+        hipEventRecord(c->ev_free[1], c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    const size_t fb = frame * sizeof(float);
+    for (int b = 0; b < n_blocks; b++) {
+        const int k = b & 1;
+        float *dst = c->pull[k].as<float>();
+        if (hipStreamWaitEvent(c->copy_stream, c->ev_free[k], 0) != hipSuccess) return CTBB_ERR_HIP;
+        // readings [start, start + count): the part shared with the previous
+        // pull comes from the other buffer (device to device), the rest from
+        // the host scan, and readings outside the scan are zeros
+        long lo_prev = 0, hi_prev = 0;
+        if (b > 0) {
+            lo_prev = std::max(start[b], start[b - 1]);
+            hi_prev = std::min(start[b] + count[b], start[b - 1] + count[b - 1]);
+        }
+        if (hi_prev > lo_prev &&
+            hipMemcpyAsync(dst + static_cast<size_t>(lo_prev - start[b]) * frame,
+                           c->pull[k ^ 1].as<float>() + static_cast<size_t>(lo_prev - start[b - 1]) * frame,
+                           static_cast<size_t>(hi_prev - lo_prev) * fb, hipMemcpyDeviceToDevice,
+                           c->copy_stream) != hipSuccess)
+            return CTBB_ERR_HIP;
+        // the rest: [start, lo_prev) and [max(hi_prev, start), end) in reading space
+        const long seg[2][2] = {{start[b], hi_prev > lo_prev ? lo_prev : start[b] + count[b]},
+                                {hi_prev > lo_prev ? hi_prev : start[b] + count[b], start[b] + count[b]}};
+        for (const auto &sg : seg) {
+            const long a0 = sg[0], a1 = sg[1];
+            if (a1 <= a0) continue;
+            const long in0 = std::max<long>(a0, 0), in1 = std::min<long>(a1, static_cast<long>(n_raw_readings));
+            if (in0 > a0) {
+                st = fill_zero(dst + static_cast<size_t>(a0 - start[b]) * frame,
+                               static_cast<size_t>(std::min(in0, a1) - a0) * frame, c->copy_stream);
+                if (st) return st;
+            }
+            if (in1 > in0 &&
+                hipMemcpyAsync(dst + static_cast<size_t>(in0 - start[b]) * frame, h_raw + static_cast<size_t>(in0) * frame,
+                               static_cast<size_t>(in1 - in0) * fb, hipMemcpyHostToDevice, c->copy_stream) != hipSuccess)
+                return CTBB_ERR_HIP;
+            const long z0 = std::max(in1, a0);
+            if (a1 > z0) {
+                st = fill_zero(dst + static_cast<size_t>(z0 - start[b]) * frame, static_cast<size_t>(a1 - z0) * frame,
+                               c->copy_stream);
+                if (st) return st;
+            }
+        }
+        if (hipEventRecord(c->ev_ready[k], c->copy_stream) != hipSuccess ||
+            hipStreamWaitEvent(c->stream, c->ev_ready[k], 0) != hipSuccess)
+            return CTBB_ERR_HIP;
+        ctbb::plan_block(mr, b);
+        st = process_block(c, mr, b, dst, c->vtab.as<BpView>() + off[b], d_image);
+        if (st) return st;
+        // buffer k is free once this block's compute is done; the next block
+        // (other buffer) copies its overlap out of it, so that copy must also
+        // finish before buffer k is refilled: both are ordered on copy_stream
+        if (hipEventRecord(c->ev_free[k], c->stream) != hipSuccess) return CTBB_ERR_HIP;
+    }
+    const size_t final_bytes = static_cast<size_t>(ctbb_n_slices_final(mr)) * nxy * sizeof(float);
+    st = c->raw_in.ensure(final_bytes);
+    if (st) return st;
+    st = do_finalize(c, mr, d_image, c->raw_in.as<float>());
+    if (st) return st;
+    if (hipMemcpyAsync(h_final, c->raw_in.p, final_bytes, hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    return hip_ok(hipStreamSynchronize(c->stream));
 }
 
 // --- reference host-memory contract ---------------------------------------

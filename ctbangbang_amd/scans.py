@@ -100,8 +100,9 @@ def mini_install(dirpath, params=None):
     return dirpath
 
 
-def mini_tokens(ffs="none", nrows=8, nx=32, table_dir="out", rotations=None):
-    """Small scan on the 128-channel scanner: ffs in none/phi/z/both."""
+def mini_tokens(ffs="none", nrows=8, nx=32, table_dir="out", rotations=None, n_slices=4):
+    """Small scan on the 128-channel scanner: ffs in none/phi/z/both;
+    n_slices final slices of 2.4 mm centred on the scan."""
     t = base_tokens()
     phi = 1 if ffs in ("phi", "both") else 0
     z = 1 if ffs in ("z", "both") else 0
@@ -113,9 +114,9 @@ def mini_tokens(ffs="none", nrows=8, nx=32, table_dir="out", rotations=None):
               "PitchValue": feed, "Readings": rot * 192 * n_ffs, "Nx": nx, "Ny": nx, "ReconFOV": 250.0,
               "SliceThickness": 2.4, "TableDir": table_dir, "AdaptiveFiltration": 1.0, "MuWater": 0.02})
     length = rot * feed
-    t["StartPos"] = round(0.5 * length - 8.0, 3)
-    t["NSlices"] = 4
-    t["EndPos"] = t["StartPos"] + 3 * 2.4
+    t["StartPos"] = round(0.5 * length - 8.0 - 1.2 * (n_slices - 4), 3)
+    t["NSlices"] = n_slices
+    t["EndPos"] = t["StartPos"] + (n_slices - 1) * 2.4
     return t
 
 

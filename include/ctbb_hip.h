@@ -232,6 +232,18 @@ int ctbb_reconstruct_dev(ctbb_ctx *ctx, struct recon_metadata *mr,
                          const float *d_raw, size_t n_raw_readings,
                          float *d_image, float *d_final);
 
+/* Whole reconstruction from a scan in host memory: the same pipeline as
+ * ctbb_reconstruct_dev, with each block's raw pull assembled in one of two
+ * device buffers on a copy stream (readings shared with the previous pull
+ * copied device to device, the new ones host to device, readings outside
+ * [0, n_raw_readings) zero) while the previous block is reconstructed.  The
+ * final stack (n_slices_final*nx*ny floats) is copied to h_final and the call
+ * returns when it is there.  Pass page-locked h_raw (hipHostMalloc) for the
+ * copies to overlap the compute.  No counterpart in the reference, whose
+ * driver reads and uploads every pull whole (src/setup.cu:605-767). */
+int ctbb_reconstruct_host(ctbb_ctx *ctx, struct recon_metadata *mr,
+                          const float *h_raw, size_t n_raw_readings, float *h_final);
+
 /* ------------------------------------------------------------------------ */
 /* Synthetic helical scans (test data generator).                           */
 /* ------------------------------------------------------------------------ */
