@@ -192,7 +192,9 @@ def test_backproject_random_sinogram(ctx, tmp_path, mini_dir, table_dir, nz):
 @pytest.mark.parametrize("nz", [32, 16])
 def test_backproject_definitionas_block_sampled(ctx, tmp_path, nz):
     """Full-size phi+z FFS block (512^2, 64 rows): HIP vs FP64 oracle on 400
-    sampled voxel columns of a filtered synthetic scan, in HU."""
+    sampled voxel columns plus the four 24x24 corner patches (voxels that
+    the views of an angle see only at the detector edge, |q| -> 1, where W is
+    tiny but h_t must stay non-zero) of a filtered synthetic scan, in HU."""
     import oracle
     tok = scans.c1_tokens()
     mr = _mr(tmp_path, tok, None)
@@ -208,14 +210,29 @@ def test_backproject_definitionas_block_sampled(ctx, tmp_path, nz):
     ctx.set_options(bp_tile=0)
     rng = np.random.default_rng(11)
     idx = rng.choice(mr.rp.nx * mr.rp.ny, 400, replace=False)
-    cx, cy = idx // mr.rp.ny, idx % mr.rp.ny
+    cx, cy = list(idx // mr.rp.ny), list(idx % mr.rp.ny)
+    for x0 in (0, mr.rp.nx - 24):
+        for y0 in (0, mr.rp.ny - 24):
+            for x in range(x0, x0 + 24):
+                for y in range(y0, y0 + 24):
+                    cx.append(x)
+                    cy.append(y)
+    cx, cy = np.array(cx), np.array(cy)
     want = oracle.backproject(mr, host(reb).reshape(s.n_proj_out, mr.cg.n_rows, -1), oracle.BP_GPU,
                               cols=(cx, cy), double=True)
     g = got[:, cx, cy].astype(np.float64)
     w = want[:, cx, cy].astype(np.float64)
     hu = 1000.0 * (g - w) / 0.02
-    print(f"bp c1 block (nz={nz}): HU max {np.abs(hu).max():.4f} rms {np.sqrt(np.mean(hu ** 2)):.4f}")
-    assert np.abs(hu).max() <= 1.0
+    n_off = int((np.abs(hu) > 0.05).sum())
+    print(f"bp c1 block (nz={nz}): HU max {np.abs(hu).max():.4f} rms {np.sqrt(np.mean(hu ** 2)):.4f}, "
+          f"{n_off} of {hu.size} voxels beyond 0.05 HU")
+    # W(q) jumps from ~8e-10 to 0 at |q| = 1 and h_t renormalises, so a voxel
+    # that one view of an angle sees at |q| within rounding of 1 takes that
+    # sample in full or not at all: rare single-voxel steps are inherent to
+    # the weighting at FP32 geometry, anything systematic is not.
+    assert np.sqrt(np.mean(hu ** 2)) <= 0.02
+    assert np.abs(hu).max() <= 2.0
+    assert n_off <= 1e-4 * hu.size
     cb.free_metadata(mr)
 
 
