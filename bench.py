@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--stage", default="all", choices=["all", "backproject", "copy"],
                     help="all: whole volume (the benchmark); backproject: only the backprojection of one block "
                          "(for counter runs); copy: a 1 GiB device copy (HBM counter calibration)")
-    ap.add_argument("--bp-tile", type=int, default=32, choices=[16, 32])
+    ap.add_argument("--bp-tile", type=int, default=16, choices=[16, 32])
     ap.add_argument("--filter", default=None, choices=["fft32", "f32", "bf16x3", "f16x3"],
                     help="ramp filter form (default: fft32 for --form plain, f16x3 for --form mixed)")
     return ap.parse_args()

@@ -104,8 +104,15 @@ static_assert(sizeof(BpView) == 16 && alignof(BpView) <= 16, "view table entries
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
-template <int NZ, int CPT, int TPR, bool kBf16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bp_kernel(BpArgs a) {
+// ZS z-groups of NZ slices share one workgroup (256 * ZS threads): thread t
+// takes column t % 256 and slices (t / 256) * NZ ... + NZ - 1 of the block.
+// NZ = 32, ZS = 1 keeps 96 accumulators per thread (150 VGPRs, 3 waves per
+// SIMD); NZ = 16, ZS = 2 halves them (about 95 VGPRs, 4 waves per SIMD) and
+// shares one staged window between the two halves.
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 1 ? 3 : 4))) void bp_kernel(
+    BpArgs a) {
+    constexpr int THREADS = 256 * ZS;
     // dynamic LDS: 2 window buffers x [rows_pad + 1][RS] pairs, then a ring
     // of 128 view-table entries
     extern __shared__ float2 lds2[];
@@ -119,9 +126,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
         tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     }
     const int tx = tile / a.tiles_y, ty = tile - tx * a.tiles_y;
-    const int xi = tx * 16 + (tid >> 4);
-    const int yi = ty * 16 + (tid & 15);
-    const int z0 = blockIdx.y * NZ;
+    const int col = tid & 255;
+    const int xi = tx * 16 + (col >> 4);
+    const int yi = ty * 16 + (col & 15);
+    const int z0 = (blockIdx.y * ZS + (tid >> 8)) * NZ;
 
     const float x = a.xs * (static_cast<float>(xi) - a.xc) + a.x0;
     const float y = a.ys * (static_cast<float>(yi) - a.yc) + a.y0;
@@ -131,7 +139,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
 
     const int tpr = TPR ? TPR : a.tpr;
     const int tpr_log2 = TPR ? ilog2(TPR) : a.tpr_log2;
-    const int rows_pad = 256 >> tpr_log2;
+    const int rows_pad = THREADS >> tpr_log2;
     const int W = tpr * CPT;                            // staged channels
     const int RS = tpr * (CPT + 1);                     // pairs per staged row
     const int n_stage = (rows_pad + 1) * RS;            // pairs per buffer
@@ -140,7 +148,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
     const bool srow_ok = srow < a.n_rows;
 
     // the zero row (row rows_pad) of both buffers
-    for (int c = tid; c < RS; c += 256) {
+    for (int c = tid; c < RS; c += THREADS) {
         lds2[rows_pad * RS + c] = make_float2(0.0f, 0.0f);
         lds2[n_stage + rows_pad * RS + c] = make_float2(0.0f, 0.0f);
     }
@@ -310,59 +318,88 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
     for (int z = 0; z < NZ; z++) o[z * nxy] = in_fov ? s[z] * a.out_scale : 0.0f;
 }
 
-template <int NZ, int CPT, int TPR>
+template <int NZ, int CPT, int TPR, int ZS>
 int launch_c(const BpArgs &a, hipStream_t stream) {
-    const int rows_pad = 256 >> a.tpr_log2;
+    const int rows_pad = (256 * ZS) >> a.tpr_log2;
     const size_t lds_bytes = 2 * static_cast<size_t>(rows_pad + 1) * a.tpr * (CPT + 1) * sizeof(float2) +
                              128 * sizeof(float4);
-    if (lds_bytes > 65536) return CTBB_ERR_SHAPE;
-    dim3 grid(a.n_tiles, 32 / NZ);
-    if (a.rebin_bf16) hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, true>), grid, dim3(256), lds_bytes, stream, a);
-    else hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, false>), grid, dim3(256), lds_bytes, stream, a);
+    auto kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, true>)
+                           : reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, false>);
+    if (lds_bytes > 65536) {
+        // tall detectors: up to the device's per-workgroup LDS
+        int dev = 0, max_lds = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return CTBB_ERR_HIP;
+        if (lds_bytes > static_cast<size_t>(max_lds)) return CTBB_ERR_SHAPE;
+        if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes)) !=
+            hipSuccess)
+            return CTBB_ERR_SHAPE;
+    }
+    dim3 grid(a.n_tiles, 32 / (NZ * ZS));
+    if (a.rebin_bf16)
+        hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, ZS, true>), grid, dim3(256 * ZS), lds_bytes, stream, a);
+    else
+        hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, ZS, false>), grid, dim3(256 * ZS), lds_bytes, stream, a);
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 
-// Channels per thread for one slices-per-thread variant: the smallest
-// supported CPT whose window W = tpr * CPT covers the tile footprint
-// (2 half_w + 3 channels).  The (TPR, CPT) shapes of 16-, 32- and 64-row
-// detectors have kernels with compile-time strides; others use the generic one.
-template <int NZ>
+// Channels per thread: the smallest supported CPT whose window
+// W = tpr * CPT covers the tile footprint (2 half_w + 3 channels).  The
+// (TPR, CPT) shapes of 16-, 32- and 64-row detectors at the usual fields of
+// view have kernels with compile-time strides; others use the generic one.
+template <int NZ, int ZS>
 int launch_nz(BpArgs a, hipStream_t stream) {
+    constexpr int kThreads = 256 * ZS;
     const float need = 2.0f * a.half_w + 3.0f;
     int rows_pad = 2;
     while (rows_pad < a.n_rows) rows_pad <<= 1;
-    if (rows_pad > 256) return CTBB_ERR_SHAPE;
-    a.tpr = 256 / rows_pad;
+    if (rows_pad > kThreads) return CTBB_ERR_SHAPE;
+    a.tpr = kThreads / rows_pad;
     a.tpr_log2 = 0;
     while ((1 << a.tpr_log2) < a.tpr) a.tpr_log2++;
     const int cpt = static_cast<int>(ceilf(need / static_cast<float>(a.tpr)));
-    static const int kCpt[] = {2, 4, 6, 8, 10, 12, 14};
+    static const int kCpt[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 14};
     for (int c : kCpt) {
         if (c < cpt) continue;
-        if constexpr (NZ == 32) {
-            const int key = a.tpr * 100 + c;
+        const int key = a.tpr * 100 + c;
+        if constexpr (ZS == 1) {
             switch (key) {
-                case 406: return launch_c<NZ, 6, 4>(a, stream);
-                case 408: return launch_c<NZ, 8, 4>(a, stream);
-                case 410: return launch_c<NZ, 10, 4>(a, stream);
-                case 412: return launch_c<NZ, 12, 4>(a, stream);
-                case 414: return launch_c<NZ, 14, 4>(a, stream);
-                case 804: return launch_c<NZ, 4, 8>(a, stream);
-                case 806: return launch_c<NZ, 6, 8>(a, stream);
-                case 808: return launch_c<NZ, 8, 8>(a, stream);
-                case 1602: return launch_c<NZ, 2, 16>(a, stream);
-                case 1604: return launch_c<NZ, 4, 16>(a, stream);
+                case 406: return launch_c<NZ, 6, 4, ZS>(a, stream);
+                case 408: return launch_c<NZ, 8, 4, ZS>(a, stream);
+                case 410: return launch_c<NZ, 10, 4, ZS>(a, stream);
+                case 412: return launch_c<NZ, 12, 4, ZS>(a, stream);
+                case 414: return launch_c<NZ, 14, 4, ZS>(a, stream);
+                case 804: return launch_c<NZ, 4, 8, ZS>(a, stream);
+                case 806: return launch_c<NZ, 6, 8, ZS>(a, stream);
+                case 808: return launch_c<NZ, 8, 8, ZS>(a, stream);
+                case 1602: return launch_c<NZ, 2, 16, ZS>(a, stream);
+                case 1604: return launch_c<NZ, 4, 16, ZS>(a, stream);
+                default: break;
+            }
+        } else {
+            switch (key) {
+                case 804: return launch_c<NZ, 4, 8, ZS>(a, stream);
+                case 805: return launch_c<NZ, 5, 8, ZS>(a, stream);
+                case 806: return launch_c<NZ, 6, 8, ZS>(a, stream);
+                case 1602: return launch_c<NZ, 2, 16, ZS>(a, stream);
+                case 1603: return launch_c<NZ, 3, 16, ZS>(a, stream);
+                case 3201: return launch_c<NZ, 1, 32, ZS>(a, stream);
+                case 3202: return launch_c<NZ, 2, 32, ZS>(a, stream);
                 default: break;
             }
         }
         switch (c) {
-            case 2: return launch_c<NZ, 2, 0>(a, stream);
-            case 4: return launch_c<NZ, 4, 0>(a, stream);
-            case 6: return launch_c<NZ, 6, 0>(a, stream);
-            case 8: return launch_c<NZ, 8, 0>(a, stream);
-            case 10: return launch_c<NZ, 10, 0>(a, stream);
-            case 12: return launch_c<NZ, 12, 0>(a, stream);
-            case 14: return launch_c<NZ, 14, 0>(a, stream);
+            case 1: return launch_c<NZ, 1, 0, ZS>(a, stream);
+            case 2: return launch_c<NZ, 2, 0, ZS>(a, stream);
+            case 3: return launch_c<NZ, 3, 0, ZS>(a, stream);
+            case 4: return launch_c<NZ, 4, 0, ZS>(a, stream);
+            case 5: return launch_c<NZ, 5, 0, ZS>(a, stream);
+            case 6: return launch_c<NZ, 6, 0, ZS>(a, stream);
+            case 8: return launch_c<NZ, 8, 0, ZS>(a, stream);
+            case 10: return launch_c<NZ, 10, 0, ZS>(a, stream);
+            case 12: return launch_c<NZ, 12, 0, ZS>(a, stream);
+            case 14: return launch_c<NZ, 14, 0, ZS>(a, stream);
         }
     }
     return CTBB_ERR_SHAPE;   // footprint wider than 14 channels per thread
@@ -375,8 +412,8 @@ int launch_backproject(const BpArgs &a, hipStream_t stream) {
     if (a.nx % 16 || a.ny % 16 || a.nx <= 0 || a.ny <= 0) return CTBB_ERR_SHAPE;
     if (a.n_rows < 2 || a.half <= 0 || a.n_ht <= 0) return CTBB_ERR_SHAPE;
     if (a.n_tiles != (a.nx / 16) * (a.ny / 16) || a.tiles_y != a.ny / 16) return CTBB_ERR_ARG;
-    if (a.nz == 32) return launch_nz<32>(a, stream);
-    if (a.nz == 16) return launch_nz<16>(a, stream);
+    if (a.nz == 32) return launch_nz<32, 1>(a, stream);
+    if (a.nz == 16) return launch_nz<16, 2>(a, stream);
     return CTBB_ERR_ARG;
 }
 

@@ -291,7 +291,7 @@ int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin,
     BpArgs a{};
     int st = fill_bp_args(mr, a);
     if (st) return st;
-    a.nz = ctx->opt.bp_tile == 16 ? 16 : 32;
+    a.nz = ctx->opt.bp_tile == 32 ? 32 : 16;
     if (ctx->opt.proj_dtype == CTBB_PROJ_BF16) a.rebin_bf16 = static_cast<const uint16_t *>(d_rebin);
     else a.rebin = static_cast<const float *>(d_rebin);
     a.vtab = d_vtab;
