@@ -43,6 +43,7 @@ struct DevBuf {
 
 struct EventPair {
     hipEvent_t a, b;
+    hipStream_t s;
 };
 
 }  // namespace
@@ -58,13 +59,19 @@ struct ctbb_ctx {
     DevBuf pull[2];
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    // whole-scan pipelines: block b + 1 is prepared (adaptive filter, rebin,
+    // ramp filter) on `stream` while block b is backprojected on `stream_bp`,
+    // with two filtered-projection buffers
+    hipStream_t stream_bp = nullptr;
+    DevBuf rebin_out2;
+    hipEvent_t ev_filt[2] = {nullptr, nullptr}, ev_bp[2] = {nullptr, nullptr}, ev_sync = nullptr;
     // cached host-side keys
     std::string taps_key, maps_key;
     int map_kind = 0, map_sets = 0;
     std::map<std::string, std::vector<EventPair>> events;
     std::vector<EventPair> free_events;
 
-    EventPair begin(const char *family) {
+    EventPair begin(const char *family, hipStream_t on = nullptr) {
         EventPair e;
         if (!free_events.empty()) {
             e = free_events.back();
@@ -73,11 +80,12 @@ struct ctbb_ctx {
             hipEventCreate(&e.a);
             hipEventCreate(&e.b);
         }
-        hipEventRecord(e.a, stream);
+        e.s = on ? on : stream;
+        hipEventRecord(e.a, e.s);
         events[family].push_back(e);
         return e;
     }
-    void end(const EventPair &e) { hipEventRecord(e.b, stream); }
+    void end(const EventPair &e) { hipEventRecord(e.b, e.s); }
     void reset_events() {
         for (auto &kv : events)
             for (auto &e : kv.second) free_events.push_back(e);
@@ -287,7 +295,8 @@ int fill_bp_args(const recon_metadata *mr, BpArgs &a) {
 }
 
 int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin, const BpView *d_vtab,
-                   float *d_out) {
+                   float *d_out, hipStream_t on = nullptr) {
+    if (!on) on = ctx->stream;
     BpArgs a{};
     int st = fill_bp_args(mr, a);
     if (st) return st;
@@ -296,8 +305,8 @@ int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin,
     else a.rebin = static_cast<const float *>(d_rebin);
     a.vtab = d_vtab;
     a.out = d_out;
-    EventPair e = ctx->begin("backproject");
-    st = launch_backproject(a, ctx->stream);
+    EventPair e = ctx->begin("backproject", on);
+    st = launch_backproject(a, on);
     ctx->end(e);
     return st;
 }
@@ -422,12 +431,10 @@ int backproject_dev(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin
     return do_backproject(ctx, mr, d_rebin, ctx->vtab.as<BpView>(), d_img);
 }
 
-// Adaptive filter -> rebin -> ramp filter -> backprojection of block b of
-// the plan (mr->ri.cb set to it), raw pull at `win` on the device.
-int process_block(ctbb_ctx *c, recon_metadata *mr, int b, const float *win, const BpView *d_vtab,
-                  float *d_image) {
+// Adaptive filter -> rebin -> ramp filter of the current block (mr->ri.cb)
+// on ctx->stream, raw pull at `win`, filtered projections into d_out.
+int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out) {
     const ct_geom &cg = mr->cg;
-    const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
     const size_t frame = cg.n_rows_raw * cg.n_channels;
     const size_t n = mr->ri.n_proj_pull;
     ctbb_sizes sz;
@@ -444,13 +451,54 @@ int process_block(ctbb_ctx *c, recon_metadata *mr, int b, const float *win, cons
     if (st) return st;
     st = do_rebin(c, mr, win, c->rebin_tmp.as<float>());
     if (st) return st;
-    st = c->rebin_out.ensure(sz.rebin * proj_elem_bytes(c));
+    return do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), sz.n_proj_out * cg.n_rows,
+                     c->rebin_tmp.as<float>(), d_out, c->opt.proj_dtype == CTBB_PROJ_BF16);
+}
+
+int ensure_pipeline(ctbb_ctx *c, size_t rebin_bytes) {
+    if (!c->stream_bp) {
+        if (hipStreamCreateWithFlags(&c->stream_bp, hipStreamNonBlocking) != hipSuccess) return CTBB_ERR_HIP;
+        for (int i = 0; i < 2; i++)
+            if (hipEventCreateWithFlags(&c->ev_filt[i], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming) != hipSuccess)
+                return CTBB_ERR_HIP;
+        if (hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming) != hipSuccess) return CTBB_ERR_HIP;
+    }
+    int st = c->rebin_out.ensure(rebin_bytes);
     if (st) return st;
-    st = do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), sz.n_proj_out * cg.n_rows,
-                   c->rebin_tmp.as<float>(), c->rebin_out.p, c->opt.proj_dtype == CTBB_PROJ_BF16);
+    st = c->rebin_out2.ensure(rebin_bytes);
     if (st) return st;
-    return do_backproject(c, mr, c->rebin_out.p, d_vtab,
-                          d_image + static_cast<size_t>(b) * mr->ri.n_slices_block * nxy);
+    // the backprojection stream starts after everything queued so far on ctx->stream
+    if (hipEventRecord(c->ev_sync, c->stream) != hipSuccess ||
+        hipStreamWaitEvent(c->stream_bp, c->ev_sync, 0) != hipSuccess)
+        return CTBB_ERR_HIP;
+    return CTBB_OK;
+}
+
+// Block b of a whole-scan pipeline: prepare on ctx->stream into filtered-
+// projection buffer b % 2 (after the backprojection of block b - 2 has read
+// it), then backproject on stream_bp once the preparation is done.
+int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int b, const float *win, const BpView *d_vtab,
+                   float *d_image) {
+    const int k = b & 1;
+    DevBuf &out = k ? c->rebin_out2 : c->rebin_out;
+    if (b >= 2 && hipStreamWaitEvent(c->stream, c->ev_bp[k], 0) != hipSuccess) return CTBB_ERR_HIP;
+    int st = prep_block(c, mr, win, out.p);
+    if (st) return st;
+    if (hipEventRecord(c->ev_filt[k], c->stream) != hipSuccess ||
+        hipStreamWaitEvent(c->stream_bp, c->ev_filt[k], 0) != hipSuccess)
+        return CTBB_ERR_HIP;
+    const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
+    st = do_backproject(c, mr, out.p, d_vtab, d_image + static_cast<size_t>(b) * mr->ri.n_slices_block * nxy,
+                        c->stream_bp);
+    if (st) return st;
+    return hip_ok(hipEventRecord(c->ev_bp[k], c->stream_bp));
+}
+
+// ctx->stream waits for every backprojection queued on stream_bp
+int join_pipeline(ctbb_ctx *c) {
+    if (hipEventRecord(c->ev_sync, c->stream_bp) != hipSuccess) return CTBB_ERR_HIP;
+    return hip_ok(hipStreamWaitEvent(c->stream, c->ev_sync, 0));
 }
 
 }  // namespace
@@ -487,14 +535,21 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
+    if (c->stream_bp) hipStreamSynchronize(c->stream_bp);
     for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->rebin_out, &c->image_block, &c->image_stack,
-                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1]})
+                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1], &c->rebin_out2})
         b->release();
     for (int i = 0; i < 2; i++) {
         if (c->ev_ready[i]) hipEventDestroy(c->ev_ready[i]);
         if (c->ev_free[i]) hipEventDestroy(c->ev_free[i]);
     }
     if (c->copy_stream) hipStreamDestroy(c->copy_stream);
+    for (int i = 0; i < 2; i++) {
+        if (c->ev_filt[i]) hipEventDestroy(c->ev_filt[i]);
+        if (c->ev_bp[i]) hipEventDestroy(c->ev_bp[i]);
+    }
+    if (c->ev_sync) hipEventDestroy(c->ev_sync);
+    if (c->stream_bp) hipStreamDestroy(c->stream_bp);
     c->reset_events();
     for (auto &e : c->free_events) {
         hipEventDestroy(e.a);
@@ -522,12 +577,13 @@ void *ctbb_ctx_stream(ctbb_ctx *c) { return c ? static_cast<void *>(c->stream) :
 
 int ctbb_ctx_synchronize(ctbb_ctx *c) {
     if (!c) return CTBB_ERR_ARG;
+    if (c->stream_bp && hipStreamSynchronize(c->stream_bp) != hipSuccess) return CTBB_ERR_HIP;
     return hip_ok(hipStreamSynchronize(c->stream));
 }
 
 int ctbb_ctx_kernel_time(ctbb_ctx *c, const char *family, float *total_ms, int *launches) {
     if (!c || !family || !total_ms || !launches) return CTBB_ERR_ARG;
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return CTBB_ERR_HIP;
+    if (ctbb_ctx_synchronize(c) != CTBB_OK) return CTBB_ERR_HIP;
     *total_ms = 0.0f;
     *launches = 0;
     auto it = c->events.find(family);
@@ -632,6 +688,15 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
     if (st) return st;
     st = ensure_maps(c, mr);
     if (st) return st;
+    size_t max_rebin = 0;
+    for (int b = 0; b < n_blocks; b++) {
+        ctbb::plan_block(mr, b);
+        ctbb_sizes sz;
+        ctbb_block_sizes(mr, &sz);
+        max_rebin = std::max(max_rebin, sz.rebin);
+    }
+    st = ensure_pipeline(c, max_rebin * proj_elem_bytes(c));
+    if (st) return st;
     for (int b = 0; b < n_blocks; b++) {
         ctbb::plan_block(mr, b);
         ctbb_sizes sz;
@@ -657,9 +722,11 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
                 return CTBB_ERR_HIP;
             win = dst;
         }
-        st = process_block(c, mr, b, win, c->vtab.as<BpView>() + off[b], d_image);
+        st = pipeline_block(c, mr, b, win, c->vtab.as<BpView>() + off[b], d_image);
         if (st) return st;
     }
+    st = join_pipeline(c);
+    if (st) return st;
     return do_finalize(c, mr, d_image, d_final);
 }
 
@@ -709,6 +776,15 @@ int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h
         st = c->pull[i].ensure(max_pull * frame * sizeof(float));
         if (st) return st;
     }
+    size_t max_rebin = 0;
+    for (int b = 0; b < n_blocks; b++) {
+        ctbb::plan_block(mr, b);
+        ctbb_sizes sz;
+        ctbb_block_sizes(mr, &sz);
+        max_rebin = std::max(max_rebin, sz.rebin);
+    }
+    st = ensure_pipeline(c, max_rebin * proj_elem_bytes(c));
+    if (st) return st;
     // the copy stream must not overwrite a buffer before earlier work on the
     // compute stream (a previous call) is done with it
     if (hipEventRecord(c->ev_free[0], c->stream) != hipSuccess ||
@@ -760,13 +836,16 @@ int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h
             hipStreamWaitEvent(c->stream, c->ev_ready[k], 0) != hipSuccess)
             return CTBB_ERR_HIP;
         ctbb::plan_block(mr, b);
-        st = process_block(c, mr, b, dst, c->vtab.as<BpView>() + off[b], d_image);
+        st = pipeline_block(c, mr, b, dst, c->vtab.as<BpView>() + off[b], d_image);
         if (st) return st;
-        // buffer k is free once this block's compute is done; the next block
-        // (other buffer) copies its overlap out of it, so that copy must also
-        // finish before buffer k is refilled: both are ordered on copy_stream
+        // the pull in buffer k has been read once this block's preparation
+        // on ctx->stream is done (the backprojection reads the filtered
+        // projections); the next block copies its overlap out of it, so that
+        // copy is also ordered before buffer k is refilled (copy_stream)
         if (hipEventRecord(c->ev_free[k], c->stream) != hipSuccess) return CTBB_ERR_HIP;
     }
+    st = join_pipeline(c);
+    if (st) return st;
     const size_t final_bytes = static_cast<size_t>(ctbb_n_slices_final(mr)) * nxy * sizeof(float);
     st = c->raw_in.ensure(final_bytes);
     if (st) return st;
