@@ -28,7 +28,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c4"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3", "c4"],
+                    help="c1 (default): one phi+z FFS scan; c3: the batching workload (many scans)")
+    ap.add_argument("--c3-scans", type=int, default=32, help="number of scans in the c3 batch")
     ap.add_argument("--form", default="plain", choices=["plain", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host-ingest", action="store_true",
@@ -90,6 +92,71 @@ def stage_backproject(args, cb, scans, mr, ctx, raw, frame):
                       "bp_tile": args.bp_tile, "form": args.form}), flush=True)
 
 
+def bench_c3(args):
+    """Batching form: --c3-scans synthetic scans of uneven length, pitch, FOV
+    and matrix size, reconstructed (a) one reconstruct_dev call per scan and
+    (b) as one reconstruct_batch_dev pipeline; value = voxel-updates/s of (b)."""
+    import numpy as np
+    import torch
+
+    import ctbangbang_amd as cb
+    from ctbangbang_amd import scans
+    d = tempfile.mkdtemp()
+    ctx = cb.Context(0)
+    ctx.set_options(filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32, mu_water=0.02, adaptive_filter=1,
+                    bp_tile=args.bp_tile)
+    mrs, raws, nreads, finals, vu = [], [], [], [], 0.0
+    for i in range(args.c3_scans):
+        tok = scans.c3_tokens(i)
+        prm = os.path.join(d, f"c3_{i}.prm")
+        cb.write_prm(prm, tok)
+        mr = cb.metadata_from_prm(prm)
+        for b in range(mr.ri.n_blocks):
+            cb.plan_block(mr, b)
+            vu += cb.block_sizes(mr).voxel_updates_block
+        n_read = int(tok["Readings"])
+        frame = mr.cg.n_rows_raw * mr.cg.n_channels
+        raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda:0")
+        ctx.synth_scan_dev(mr, scans.phantom_for(tok, 3), 0, n_read, raw)
+        mrs.append(mr)
+        raws.append(raw)
+        nreads.append(n_read)
+        finals.append(torch.empty(cb.block_sizes(mr).final_stack, dtype=torch.float32, device="cuda:0"))
+    ctx.synchronize()
+    # warm-up (allocations, tables) with the batch call
+    ctx.reconstruct_batch_dev(mrs, raws, nreads, finals)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for mr, raw, n, out in zip(mrs, raws, nreads, finals):
+        ctx.reconstruct_dev(mr, raw, n, out)
+    ctx.synchronize()
+    t_loop = time.perf_counter() - t0
+    loop_out = [finals[0].cpu().numpy().copy(), finals[-1].cpu().numpy().copy()]
+    t0 = time.perf_counter()
+    ctx.reconstruct_batch_dev(mrs, raws, nreads, finals)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    same = bool(np.array_equal(finals[0].cpu().numpy(), loop_out[0]) and
+                np.array_equal(finals[-1].cpu().numpy(), loop_out[1]))
+    n = args.c3_scans
+    line = {
+        "metric": "voxel-updates/s (batch of scans, one pipeline, scans resident in HBM)",
+        "value": vu / t_batch, "unit": "voxel-updates/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded scans: pitch 0.5-1.5, FOV 200-500 mm, 512^2/768^2, 50-400 slices)",
+        "config": {"workload": f"c3: {n} phi+z FFS scans, {vu:.4g} voxel-updates", "parallelism": "1 GPU"},
+        "scans_per_s": n / t_batch,
+        "loop": {"s": t_loop, "voxel_updates_per_s": vu / t_loop, "scans_per_s": n / t_loop},
+        "batch_over_loop": t_loop / t_batch,
+        "batch_equals_loop": same,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    for mr in mrs:
+        cb.free_metadata(mr)
+    ctx.close()
+
+
 def cpu_baseline(mr, raw_block, taps, n_cols):
     """FP32 CPU restatement (oracle/liboracle32.so, one thread) on one block:
     rebin + FFT ramp filter of the whole block and the backprojection of
@@ -130,6 +197,9 @@ def main():
         sys.exit(2)
     if args.stage == "copy":
         stage_copy(args)
+        return
+    if args.config == "c3":
+        bench_c3(args)
         return
     import torch
 

@@ -96,6 +96,8 @@ _SIGS = {
     "ctbb_finalize_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_reconstruct_dev": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP, _VP]),
     "ctbb_reconstruct_host": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP]),
+    "ctbb_reconstruct_batch_dev": (C.c_int, [_VP, C.c_int, C.POINTER(_MR), C.POINTER(_VP), C.POINTER(C.c_size_t),
+                                             C.POINTER(_VP)]),
     "ctbb_synth_scan": (C.c_int, [_MR, _P(PhantomParams), C.c_long, C.c_long, _P(C.c_float)]),
     "ctbb_synth_scan_dev": (C.c_int, [_VP, _MR, _P(PhantomParams), C.c_long, C.c_long, _VP]),
 }
@@ -303,6 +305,17 @@ class Context:
         overlaps the uploads with the compute); h_final is host memory."""
         _check(lib().ctbb_reconstruct_host(self._h, C.byref(mr), _ptr(h_raw), int(n_raw_readings), _ptr(h_final)),
                "reconstruct_host")
+
+    def reconstruct_batch_dev(self, mrs, d_raws, n_raw_readings, d_finals):
+        """Batching form: several device-resident scans as one pipeline
+        (returns when queued; synchronize() waits)."""
+        n = len(mrs)
+        assert len(d_raws) == n and len(n_raw_readings) == n and len(d_finals) == n
+        m_arr = (_MR * n)(*[C.pointer(m) for m in mrs])
+        r_arr = (_VP * n)(*[_ptr(d) for d in d_raws])
+        n_arr = (C.c_size_t * n)(*[int(x) for x in n_raw_readings])
+        f_arr = (_VP * n)(*[_ptr(d) for d in d_finals])
+        _check(lib().ctbb_reconstruct_batch_dev(self._h, n, m_arr, r_arr, n_arr, f_arr), "reconstruct_batch_dev")
 
     def synth_scan_dev(self, mr, pp, first, count, d_out):
         _check(lib().ctbb_synth_scan_dev(self._h, C.byref(mr), C.byref(pp), int(first), int(count), _ptr(d_out)),

@@ -63,7 +63,7 @@ struct ctbb_ctx {
     // ramp filter) on `stream` while block b is backprojected on `stream_bp`,
     // with two filtered-projection buffers
     hipStream_t stream_bp = nullptr;
-    DevBuf rebin_out2;
+    DevBuf rebin_out2, image_stack2;
     hipEvent_t ev_filt[2] = {nullptr, nullptr}, ev_bp[2] = {nullptr, nullptr}, ev_sync = nullptr;
     // cached host-side keys
     std::string taps_key, maps_key;
@@ -357,30 +357,44 @@ int finalize_table(const recon_metadata *mr, std::vector<int> &src, std::vector<
     return CTBB_OK;
 }
 
-int do_finalize(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_image, float *d_final) {
+// Finalize tap lists of one scan as one blob: [src (nf * mt ints)][w (nf * mt floats)]
+int finalize_blob(const recon_metadata *mr, std::vector<char> &blob, int *mt, int *nf) {
     std::vector<int> src;
     std::vector<float> w;
-    int mt = 0, nf = 0;
-    int st = finalize_table(mr, src, w, &mt, &nf);
+    int st = finalize_table(mr, src, w, mt, nf);
     if (st) return st;
-    std::vector<char> blob(src.size() * sizeof(int) + w.size() * sizeof(float));
+    blob.resize(src.size() * sizeof(int) + w.size() * sizeof(float));
     std::memcpy(blob.data(), src.data(), src.size() * sizeof(int));
     std::memcpy(blob.data() + src.size() * sizeof(int), w.data(), w.size() * sizeof(float));
-    st = upload(ctx, ctx->fin_tab, blob.data(), blob.size());
-    if (st) return st;
+    return CTBB_OK;
+}
+
+// Launch the finalize kernel with tap lists already on the device at d_blob.
+int launch_finalize_blob(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_image, float *d_final,
+                         const char *d_blob, int mt, int nf, hipStream_t on) {
     FinalizeArgs f{};
     f.image = d_image;
     f.out = d_final;
-    f.src_slice = ctx->fin_tab.as<int>();
-    f.weight = reinterpret_cast<const float *>(ctx->fin_tab.as<char>() + src.size() * sizeof(int));
+    f.src_slice = reinterpret_cast<const int *>(d_blob);
+    f.weight = reinterpret_cast<const float *>(d_blob + static_cast<size_t>(nf) * mt * sizeof(int));
     f.max_taps = mt;
     f.n_final = nf;
     f.nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
     f.mu_water = ctx->opt.mu_water;
-    EventPair e = ctx->begin("finalize");
-    st = launch_finalize(f, ctx->stream);
+    EventPair e = ctx->begin("finalize", on);
+    int st = launch_finalize(f, on);
     ctx->end(e);
     return st;
+}
+
+int do_finalize(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_image, float *d_final) {
+    std::vector<char> blob;
+    int mt = 0, nf = 0;
+    int st = finalize_blob(mr, blob, &mt, &nf);
+    if (st) return st;
+    st = upload(ctx, ctx->fin_tab, blob.data(), blob.size());
+    if (st) return st;
+    return launch_finalize_blob(ctx, mr, d_image, d_final, ctx->fin_tab.as<char>(), mt, nf, ctx->stream);
 }
 
 int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out) {
@@ -475,14 +489,15 @@ int ensure_pipeline(ctbb_ctx *c, size_t rebin_bytes) {
     return CTBB_OK;
 }
 
-// Block b of a whole-scan pipeline: prepare on ctx->stream into filtered-
-// projection buffer b % 2 (after the backprojection of block b - 2 has read
-// it), then backproject on stream_bp once the preparation is done.
-int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int b, const float *win, const BpView *d_vtab,
+// Item `item` of a pipeline (block b of its scan): prepare on ctx->stream into
+// filtered-projection buffer item % 2 (after the backprojection of item - 2
+// has read it), then backproject block b into d_image on stream_bp once the
+// preparation is done.
+int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int item, int b, const float *win, const BpView *d_vtab,
                    float *d_image) {
-    const int k = b & 1;
+    const int k = item & 1;
     DevBuf &out = k ? c->rebin_out2 : c->rebin_out;
-    if (b >= 2 && hipStreamWaitEvent(c->stream, c->ev_bp[k], 0) != hipSuccess) return CTBB_ERR_HIP;
+    if (item >= 2 && hipStreamWaitEvent(c->stream, c->ev_bp[k], 0) != hipSuccess) return CTBB_ERR_HIP;
     int st = prep_block(c, mr, win, out.p);
     if (st) return st;
     if (hipEventRecord(c->ev_filt[k], c->stream) != hipSuccess ||
@@ -537,7 +552,7 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
     if (c->stream_bp) hipStreamSynchronize(c->stream_bp);
     for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->rebin_out, &c->image_block, &c->image_stack,
-                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1], &c->rebin_out2})
+                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1], &c->rebin_out2, &c->image_stack2})
         b->release();
     for (int i = 0; i < 2; i++) {
         if (c->ev_ready[i]) hipEventDestroy(c->ev_ready[i]);
@@ -722,7 +737,7 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
                 return CTBB_ERR_HIP;
             win = dst;
         }
-        st = pipeline_block(c, mr, b, win, c->vtab.as<BpView>() + off[b], d_image);
+        st = pipeline_block(c, mr, b, b, win, c->vtab.as<BpView>() + off[b], d_image);
         if (st) return st;
     }
     st = join_pipeline(c);
@@ -836,7 +851,7 @@ int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h
             hipStreamWaitEvent(c->stream, c->ev_ready[k], 0) != hipSuccess)
             return CTBB_ERR_HIP;
         ctbb::plan_block(mr, b);
-        st = pipeline_block(c, mr, b, dst, c->vtab.as<BpView>() + off[b], d_image);
+        st = pipeline_block(c, mr, b, b, dst, c->vtab.as<BpView>() + off[b], d_image);
         if (st) return st;
         // the pull in buffer k has been read once this block's preparation
         // on ctx->stream is done (the backprojection reads the filtered
@@ -855,6 +870,121 @@ int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h
                        c->stream) != hipSuccess)
         return CTBB_ERR_HIP;
     return hip_ok(hipStreamSynchronize(c->stream));
+}
+
+int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *const *mrs,
+                               const float *const *d_raws, const size_t *n_raw_readings, float *const *d_finals) {
+    if (!c || n_scans <= 0 || !mrs || !d_raws || !n_raw_readings || !d_finals) return CTBB_ERR_ARG;
+    for (int s = 0; s < n_scans; s++) {
+        const recon_metadata *m = mrs[s];
+        if (!m || !d_raws[s] || !d_finals[s] || !m->table_positions || !m->tube_angles || m->ri.n_blocks <= 0)
+            return CTBB_ERR_ARG;
+    }
+    hipSetDevice(c->device);
+    // every scan must use the filter table and rebin maps of the first one
+    int st = ensure_taps(c, mrs[0]);
+    if (st) return st;
+    st = ensure_maps(c, mrs[0]);
+    if (st) return st;
+    const std::string taps0 = c->taps_key, maps0 = c->maps_key;
+    for (int s = 1; s < n_scans; s++) {
+        st = ensure_taps(c, mrs[s]);
+        if (st) return st;
+        st = ensure_maps(c, mrs[s]);
+        if (st) return st;
+        if (c->taps_key != taps0 || c->maps_key != maps0) {
+            // different scanners, kernels or focal-spot modes: one scan at a time
+            for (int t = 0; t < n_scans; t++) {
+                st = ctbb_reconstruct_dev(c, mrs[t], d_raws[t], n_raw_readings[t], nullptr, d_finals[t]);
+                if (st) return st;
+            }
+            return CTBB_OK;
+        }
+    }
+    c->reset_events();
+    // view tables of every (scan, block) and finalize tap lists of every scan,
+    // uploaded once
+    std::vector<BpView> all;
+    std::vector<std::vector<size_t>> off(n_scans);
+    std::vector<char> fin;
+    std::vector<size_t> fin_off(n_scans);
+    std::vector<int> fin_mt(n_scans), fin_nf(n_scans);
+    size_t max_rebin = 0, max_image = 0;
+    for (int s = 0; s < n_scans; s++) {
+        recon_metadata *m = mrs[s];
+        const int nb = static_cast<int>(m->ri.n_blocks);
+        off[s].resize(nb);
+        for (int b = 0; b < nb; b++) {
+            ctbb::plan_block(m, b);
+            st = check_geometry(m);
+            if (st) return st;
+            std::vector<BpView> t;
+            bp_view_table(*m, t);
+            off[s][b] = all.size();
+            all.insert(all.end(), t.begin(), t.end());
+            ctbb_sizes sz;
+            ctbb_block_sizes(m, &sz);
+            max_rebin = std::max(max_rebin, sz.rebin);
+        }
+        max_image = std::max(max_image, static_cast<size_t>(m->ri.n_slices_recon) * m->rp.nx * m->rp.ny);
+        std::vector<char> blob;
+        st = finalize_blob(m, blob, &fin_mt[s], &fin_nf[s]);
+        if (st) return st;
+        fin_off[s] = (fin.size() + 255) & ~static_cast<size_t>(255);
+        fin.resize(fin_off[s] + blob.size());
+        std::memcpy(fin.data() + fin_off[s], blob.data(), blob.size());
+    }
+    st = upload(c, c->vtab, all.data(), all.size() * sizeof(BpView));
+    if (st) return st;
+    st = upload(c, c->fin_tab, fin.data(), fin.size());
+    if (st) return st;
+    st = c->image_stack.ensure(max_image * sizeof(float));
+    if (st) return st;
+    st = c->image_stack2.ensure(max_image * sizeof(float));
+    if (st) return st;
+    st = ensure_pipeline(c, max_rebin * proj_elem_bytes(c));
+    if (st) return st;
+    // one pipeline over the (scan, block) items: preparation on ctx->stream,
+    // backprojection and then each scan's finalize on stream_bp; scans
+    // alternate between two image stacks, so finalize(s) and the
+    // backprojections of scan s + 1 never touch the same stack
+    int item = 0;
+    for (int s = 0; s < n_scans; s++) {
+        recon_metadata *m = mrs[s];
+        const int nb = static_cast<int>(m->ri.n_blocks);
+        const size_t frame = m->cg.n_rows_raw * m->cg.n_channels;
+        float *d_image = (s & 1) ? c->image_stack2.as<float>() : c->image_stack.as<float>();
+        for (int b = 0; b < nb; b++, item++) {
+            ctbb::plan_block(m, b);
+            const long start = static_cast<long>(static_cast<int>(m->ri.idx_pull_start));
+            const long n = static_cast<long>(m->ri.n_proj_pull);
+            const float *win;
+            if (start >= 0 && start + n <= static_cast<long>(n_raw_readings[s])) {
+                win = d_raws[s] + static_cast<size_t>(start) * frame;
+            } else {
+                // pull reaches outside the scan: zero-padded copy on ctx->stream
+                st = c->raw_in.ensure(static_cast<size_t>(n) * frame * sizeof(float));
+                if (st) return st;
+                float *dst = c->raw_in.as<float>();
+                st = fill_zero(dst, static_cast<size_t>(n) * frame, c->stream);
+                if (st) return st;
+                long lo = start < 0 ? 0 : start, hi = start + n;
+                if (hi > static_cast<long>(n_raw_readings[s])) hi = static_cast<long>(n_raw_readings[s]);
+                if (hi > lo && hipMemcpyAsync(dst + static_cast<size_t>(lo - start) * frame,
+                                              d_raws[s] + static_cast<size_t>(lo) * frame,
+                                              static_cast<size_t>(hi - lo) * frame * sizeof(float),
+                                              hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                    return CTBB_ERR_HIP;
+                win = dst;
+            }
+            st = pipeline_block(c, m, item, b, win, c->vtab.as<BpView>() + off[s][b], d_image);
+            if (st) return st;
+        }
+        st = launch_finalize_blob(c, m, d_image, d_finals[s], c->fin_tab.as<char>() + fin_off[s], fin_mt[s],
+                                  fin_nf[s], c->stream_bp);
+        if (st) return st;
+    }
+    return join_pipeline(c);
 }
 
 // --- reference host-memory contract ---------------------------------------

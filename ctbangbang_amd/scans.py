@@ -60,6 +60,31 @@ def c4_tokens():
     return t
 
 
+def c3_tokens(i, n_rot_max=20):
+    """Scan i of the batching workload (seed 3000 + i): phi+z FFS on the
+    64-row scanner, pitch U(0.5, 1.5), reconstruction FOV U(200, 500) mm,
+    matrix 512^2 or 768^2, U(50, 400) slices of 1 mm, with just enough
+    rotations (at most n_rot_max; fewer slices when that is not enough)."""
+    import random
+    r = random.Random(3000 + i)
+    pitch = r.uniform(0.5, 1.5)
+    fov = round(r.uniform(200.0, 500.0), 1)
+    nx = r.choice([512, 768])
+    n_slices = r.randint(50, 400)
+    feed = round(pitch * 64 * 0.6, 4)
+    # slices span (n_slices - 1) mm; the pulls need about one rotation of
+    # margin on either side of the slice range (with 0.5 extra as headroom)
+    rot = int(min(n_rot_max, max(2, -(-((n_slices - 1) + 3.0 * feed) // feed))))
+    n_slices = int(min(n_slices, max(1, (rot - 3.0) * feed + 1)))
+    start = round(1.5 * feed, 3)
+    t = base_tokens()
+    t.update({"Nrows": 64, "CollSlicewidth": 0.6, "Zffs": 1, "Phiffs": 1, "PitchValue": feed,
+              "Readings": rot * 4608, "Nx": nx, "Ny": nx, "ReconFOV": fov, "SliceThickness": 1.0,
+              "StartPos": start, "NSlices": n_slices, "EndPos": start + (n_slices - 1) * 1.0,
+              "AdaptiveFiltration": 1.0, "MuWater": 0.02})
+    return t
+
+
 def phantom_for(tokens, config_index=0, noise=True):
     """Phantom parameters: seeds 1000+config (phantom) and 2000+config (noise)."""
     pp = PhantomParams()

@@ -245,6 +245,20 @@ int ctbb_reconstruct_dev(ctbb_ctx *ctx, struct recon_metadata *mr,
 int ctbb_reconstruct_host(ctbb_ctx *ctx, struct recon_metadata *mr,
                           const float *h_raw, size_t n_raw_readings, float *h_final);
 
+/* Batching form: n_scans device-resident scans (each with its own plan in
+ * mrs[s], readings d_raws[s][0, n_raw_readings[s]) and output d_finals[s])
+ * reconstructed as one pipeline over all (scan, block) items: block
+ * preparation on the context stream overlaps the backprojection of the
+ * previous item, and each scan's finalize runs behind its last
+ * backprojection.  Scans may differ in length, pitch, FOV and matrix size;
+ * scans that do not share the first scan's filter table and rebin maps
+ * (scanner, kernel, focal-spot mode, collimation) make the call fall back to
+ * one scan at a time.  Returns when the work is queued; ctbb_ctx_synchronize
+ * waits for it.  No counterpart in the reference (one scan per process). */
+int ctbb_reconstruct_batch_dev(ctbb_ctx *ctx, int n_scans, struct recon_metadata *const *mrs,
+                               const float *const *d_raws, const size_t *n_raw_readings,
+                               float *const *d_finals);
+
 /* ------------------------------------------------------------------------ */
 /* Synthetic helical scans (test data generator).                           */
 /* ------------------------------------------------------------------------ */
