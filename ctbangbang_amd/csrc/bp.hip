@@ -160,14 +160,34 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         const float pm = (xm * vw.x - ym * vw.y) * a.inv_ds + a.two_cc;
         return static_cast<int>(floorf(pm - a.half_w)) - 1;
     };
+    // byte offset of this thread's first sample inside a projection window
+    // (row srow, channel scol)
+    constexpr unsigned kEs = kBf16 ? 2u : 4u;
+    const unsigned toff_b = static_cast<unsigned>(srow * a.nos + scol) * kEs;
     auto fetch = [&](int v, int p_lo) {
+        // v and p_lo are the same in every lane: the window base stays in
+        // scalar registers and a lane adds its 32-bit byte offset
+        v = __builtin_amdgcn_readfirstlane(v);
+        p_lo = __builtin_amdgcn_readfirstlane(p_lo);
         const int g0 = p_lo + scol;
         const size_t rb = (static_cast<size_t>(v) * a.n_rows + srow) * a.nos;
         if (srow_ok && p_lo >= 0 && p_lo + W < a.nos) {
-            // window inside the detector: CPT + 1 consecutive samples
+            // window inside the detector: CPT + 1 consecutive samples, read
+            // through a buffer descriptor of projection v (raw buffer, gfx9
+            // dword 3): per-lane offset toff_b, scalar offset p_lo, and the
+            // sample index as the instruction's immediate offset
+            const char *pb = (kBf16 ? reinterpret_cast<const char *>(a.rebin_bf16)
+                                    : reinterpret_cast<const char *>(a.rebin)) +
+                             static_cast<size_t>(v) * a.n_rows * a.nos * kEs;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<char *>(pb), static_cast<short>(0), a.n_rows * a.nos * static_cast<int>(kEs), 0x00020000);
+            const int soff = p_lo * static_cast<int>(kEs);
 #pragma unroll
             for (int e = 0; e <= CPT; e++) {
-                pv[e] = load_bits<kBf16>(a, rb + g0 + e);
+                if (kBf16)
+                    pv[e] = __builtin_amdgcn_raw_buffer_load_b16(rs, static_cast<int>(toff_b) + e * 2, soff, 0);
+                else
+                    pv[e] = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(toff_b) + e * 4, soff, 0);
                 // keep the loads separate: merged multi-dword loads land in
                 // register tuples that the (value, delta) pairs cannot use,
                 // and the copies out of them would wait for the loads here
@@ -241,7 +261,8 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         c0 = min(max(c0, 0), W - 2);
         const float xr = phat * a.inv_rf;
         float asn = asin_small(xr);
-        if (__any(fabsf(xr) > 0.45f)) asn = fabsf(xr) > 0.45f ? asinf(xr) : asn;
+        if (!a.asin_small_only && __builtin_amdgcn_ballot_w64(fabsf(xr) > 0.45f))
+            asn = fabsf(xr) > 0.45f ? asinf(xr) : asn;
         const float dz0 = vw.z + a.k_asin * asn + static_cast<float>(z0) * a.coll;
         // hardware square root (1 ulp): no denormal scaling or correction steps
         const float lhat = __builtin_amdgcn_sqrtf(a.rf2 - phat * phat) - x * vw.y - y * vw.x;

@@ -290,6 +290,10 @@ int fill_bp_args(const recon_metadata *mr, BpArgs &a) {
     const double dmax = dx > dy ? dx : dy;
     a.half_w = static_cast<float>(7.5 * std::sqrt(2.0) * dmax / ds + 1.0);
     a.acq_half = rp.acq_fov / 2.0f;
+    // largest |p| of a voxel centre: the grid corner farthest from the isocentre
+    const double hx = 0.5 * rp.recon_fov + std::fabs(static_cast<double>(rp.x_origin));
+    const double hy = 0.5 * rp.recon_fov + std::fabs(static_cast<double>(rp.y_origin));
+    a.asin_small_only = std::sqrt(hx * hx + hy * hy) <= 0.449 * cg.r_f ? 1 : 0;
     a.out_scale = static_cast<float>(2.0 * static_cast<double>(kPiBp) / cg.n_proj_turn);
     return CTBB_OK;
 }

@@ -21,6 +21,7 @@ struct BpArgs {
     float *out;                // [32][nx][ny]
     int nx, ny, nos, n_rows, half, n_ht;
     int tpr, tpr_log2;         // staging threads per detector row (set by the launcher)
+    int asin_small_only;       // 1: every voxel has |p| <= 0.45 r_f (no asinf fallback needed)
     int nz;                    // slices per thread: 32 (one z chunk) or 16 (two)
     int tiles_y;               // ny / 16
     int n_tiles;
