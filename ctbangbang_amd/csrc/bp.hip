@@ -23,8 +23,9 @@
 // W(q) = 0 for |q| >= 1 contributes exactly nothing; since q varies by less
 // than 1% across a wavefront, chunks of 4 slices that no lane sees are
 // skipped on a wave-uniform vote (about 40% of the (slice, view) pairs are
-// seen at the C1 geometry), and the cos^2 taper is evaluated only for chunks
-// where some lane is in 0.6 <= |q| < 1.  View-table entries reach the
+// seen at the C1 geometry), chunks with |q| < 0.6 in every lane take a path
+// with W = 1, and the cos^2 taper is evaluated only for slices where some
+// lane is in 0.6 <= |q| < 1.  View-table entries reach the
 // workgroup through a small LDS ring refilled every 64 views.
 //
 // Workgroups are remapped so the tiles running on one XCD are spatially
@@ -269,46 +270,62 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         const float g = a.neg_td * __builtin_amdgcn_rcpf(lhat * a.tanc);
         const float A = g * a.coll;
         const float B = g * dz0;
+        // row coordinate (q + 1) hr = Ar z + Br, used directly where |q| < 0.6
+        const float Ar = A * a.hr;
+        const float Br = fmaf(B, a.hr, a.hr);
         // LDS index of (row 0, channel c0) in this view's buffer
         const unsigned colb = static_cast<unsigned>((t & 1) * n_stage) + static_cast<unsigned>(c0) +
                               static_cast<unsigned>(c0) / CPT;
 
-        // Slices in chunks of 4.  q is linear in z, so the chunk's q range is
-        // spanned by its end slices: one wave vote skips chunks no lane sees,
-        // and a second finds chunks where every lane stays inside |q| < 0.6
-        // (W = 1 throughout, the common case).  Otherwise W is evaluated per
-        // slice, the taper only in lanes with 0.6 <= |q| < 1.  The 4 slices
-        // of a visible chunk then run without branches (W = 0 for lanes that
-        // do not see a slice, row coordinate clamped into the window), so
-        // their LDS reads are in flight together.
+        // Slices in chunks of 4.  q is linear in z, so over a chunk it stays
+        // within hw = 1.5 |A| of its value qm at the chunk centre: a chunk can
+        // be seen only if |qm| < 1 + hw, and W = 1 for all of it if
+        // |qm| < 0.6 - hw.  Both thresholds carry a margin of 1e-4, far above
+        // the rounding of q, so a skipped chunk has |q| > 1 in every lane and
+        // an interior chunk |q| < 0.6.  One wave vote skips chunks no lane
+        // sees; a second sends chunks interior for every lane (the common
+        // case) through a path with W = 1 and no clamping.  Elsewhere W is
+        // evaluated per slice, the taper only for slices where some lane has
+        // 0.6 <= |q| < 1 (a third, per-slice vote).  The 4 slices of a
+        // visible chunk then interpolate without branches (W = 0 for lanes
+        // that do not see a slice, row coordinate clamped into the window),
+        // so their LDS reads are in flight together.
+        const float hw = 1.5f * fabsf(A);
+        const float t_vis = 1.0001f + hw;
+        const float t_int = 0.5999f - hw;
         const float two_hr = 2.0f * a.hr;
 #pragma unroll
         for (int zc = 0; zc < NZ; zc += 4) {
-            const float qa = fmaf(A, static_cast<float>(zc), B);
-            const float qb = fmaf(A, static_cast<float>(zc + 3), B);
-            const float lo = fminf(qa, qb), hi = fmaxf(qa, qb);
-            if (__builtin_amdgcn_ballot_w64(lo < 1.0f && hi > -1.0f) == 0) continue;
-            float q[4], w[4];
+            const float qm = fmaf(A, static_cast<float>(zc) + 1.5f, B);
+            if (__builtin_amdgcn_ballot_w64(fabsf(qm) < t_vis) == 0) continue;
+            float rowf[4], w[4];
+            if (__builtin_amdgcn_ballot_w64(!(fabsf(qm) < t_int)) == 0) {
 #pragma unroll
-            for (int c = 0; c < 4; c++) q[c] = fmaf(A, static_cast<float>(zc + c), B);
-            if (__builtin_amdgcn_ballot_w64(lo <= -0.6f || hi >= 0.6f) == 0) {
-#pragma unroll
-                for (int c = 0; c < 4; c++) w[c] = 1.0f;
+                for (int c = 0; c < 4; c++) {
+                    rowf[c] = fmaf(Ar, static_cast<float>(zc + c), Br);
+                    w[c] = 1.0f;
+                }
             } else {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    const float aq = fabsf(q[c]);
-                    float wc = aq < 1.0f ? 1.0f : 0.0f;
-                    if (aq < 1.0f && aq >= 0.6f) wc = stier_taper(aq);
+                    const float q = fmaf(A, static_cast<float>(zc + c), B);
+                    const float aq = fabsf(q);
+                    const bool vis = aq < 1.0f;
+                    const bool band = vis && aq >= 0.6f;
+                    float wc = vis ? 1.0f : 0.0f;
+                    if (__builtin_amdgcn_ballot_w64(band)) {
+                        const float tp = stier_taper(aq);
+                        wc = band ? tp : wc;
+                    }
                     w[c] = wc;
+                    // row coordinate (q + 1) hr, in [0, n_rows - 1] where W != 0
+                    rowf[c] = __builtin_amdgcn_fmed3f(fmaf(q, a.hr, a.hr), 0.0f, two_hr);
                 }
             }
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                // row coordinate (q + 1) hr, in [0, n_rows - 1] where W != 0
-                const float rowf = __builtin_amdgcn_fmed3f(fmaf(q[c], a.hr, a.hr), 0.0f, two_hr);
-                const unsigned r0 = static_cast<unsigned>(rowf);
-                const float fr = __builtin_amdgcn_fractf(rowf);
+                const unsigned r0 = static_cast<unsigned>(rowf[c]);
+                const float fr = __builtin_amdgcn_fractf(rowf[c]);
                 const unsigned idx = colb + __umul24(r0, static_cast<unsigned>(RS));
                 const float2 e0 = lds2[idx];
                 const float2 e1 = lds2[idx + RS];
