@@ -322,15 +322,20 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
                     rowf[c] = __builtin_amdgcn_fmed3f(fmaf(q, a.hr, a.hr), 0.0f, two_hr);
                 }
             }
+            float2 e0[4], e1[4];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const unsigned r0 = static_cast<unsigned>(rowf[c]);
+                const unsigned idx = colb + __umul24(static_cast<unsigned>(rowf[c]), static_cast<unsigned>(RS));
+                e0[c] = lds2[idx];
+                e1[c] = lds2[idx + RS];
+            }
+            // all 8 reads of the chunk are issued before the first is used
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
                 const float fr = __builtin_amdgcn_fractf(rowf[c]);
-                const unsigned idx = colb + __umul24(r0, static_cast<unsigned>(RS));
-                const float2 e0 = lds2[idx];
-                const float2 e1 = lds2[idx + RS];
-                const float top = fmaf(fp, e0.y, e0.x);
-                const float bot = fmaf(fp, e1.y, e1.x);
+                const float top = fmaf(fp, e0[c].y, e0[c].x);
+                const float bot = fmaf(fp, e1[c].y, e1[c].x);
                 const float val = fmaf(fr, bot - top, top);
                 st[zc + c] = fmaf(val, w[c], st[zc + c]);
                 ht[zc + c] += w[c];
@@ -338,11 +343,12 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         }
         if (last_k) {
             // s += (1/h_t) s_t where h_t != 0 (reference include/backproject.cuh:104-108).
-            // Weights are >= 0, so h_t = 0 implies s_t = 0 and the clamp
-            // turns that case into 0 * 1e30 = 0 without a select.
+            // A nonzero weight is at least 7.8e-10, so adding 1e-30 leaves
+            // a nonzero h_t unchanged, and h_t = 0 (then s_t = 0) becomes
+            // 0 * 1e30 = 0 without a select.
 #pragma unroll
             for (int z = 0; z < NZ; z++) {
-                s[z] = fmaf(st[z], __builtin_amdgcn_rcpf(fmaxf(ht[z], 1e-30f)), s[z]);
+                s[z] = fmaf(st[z], __builtin_amdgcn_rcpf(ht[z] + 1e-30f), s[z]);
                 st[z] = 0.0f;
                 ht[z] = 0.0f;
             }

@@ -19,6 +19,10 @@
 #include "geometry.h"
 #include "kernels.h"
 
+#ifndef CTBB_PIPE_DEPTH
+#define CTBB_PIPE_DEPTH 2
+#endif
+
 namespace {
 
 struct DevBuf {
@@ -52,19 +56,20 @@ struct ctbb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     ctbb_options opt{};
-    DevBuf raw_in, raw_af, rebin_tmp, rebin_out, image_block, image_stack, af_work;
+    DevBuf raw_in, raw_af, rebin_tmp, image_block, image_stack, af_work;
     DevBuf vtab, maps, taps, spec, fin_tab;
     // host-streaming reconstruction: two raw pull buffers filled on a copy
     // stream (default priority) while the compute stream works on the other
     DevBuf pull[2];
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
-    // whole-scan pipelines: block b + 1 is prepared (adaptive filter, rebin,
-    // ramp filter) on `stream` while block b is backprojected on `stream_bp`,
-    // with two filtered-projection buffers
+    // whole-scan pipelines: the next blocks are prepared (adaptive filter,
+    // rebin, ramp filter) on `stream` while block b is backprojected on
+    // `stream_bp`, with kPipeDepth filtered-projection buffers
+    static constexpr int kPipeDepth = CTBB_PIPE_DEPTH;
     hipStream_t stream_bp = nullptr;
-    DevBuf rebin_out2, image_stack2;
-    hipEvent_t ev_filt[2] = {nullptr, nullptr}, ev_bp[2] = {nullptr, nullptr}, ev_sync = nullptr;
+    DevBuf filt[kPipeDepth], image_stack2;
+    hipEvent_t ev_filt[kPipeDepth] = {}, ev_bp[kPipeDepth] = {}, ev_sync = nullptr;
     // cached host-side keys
     std::string taps_key, maps_key;
     int map_kind = 0, map_sets = 0;
@@ -476,16 +481,16 @@ int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out) {
 int ensure_pipeline(ctbb_ctx *c, size_t rebin_bytes) {
     if (!c->stream_bp) {
         if (hipStreamCreateWithFlags(&c->stream_bp, hipStreamNonBlocking) != hipSuccess) return CTBB_ERR_HIP;
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < ctbb_ctx::kPipeDepth; i++)
             if (hipEventCreateWithFlags(&c->ev_filt[i], hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming) != hipSuccess)
                 return CTBB_ERR_HIP;
         if (hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming) != hipSuccess) return CTBB_ERR_HIP;
     }
-    int st = c->rebin_out.ensure(rebin_bytes);
-    if (st) return st;
-    st = c->rebin_out2.ensure(rebin_bytes);
-    if (st) return st;
+    for (DevBuf &b : c->filt) {
+        const int st = b.ensure(rebin_bytes);
+        if (st) return st;
+    }
     // the backprojection stream starts after everything queued so far on ctx->stream
     if (hipEventRecord(c->ev_sync, c->stream) != hipSuccess ||
         hipStreamWaitEvent(c->stream_bp, c->ev_sync, 0) != hipSuccess)
@@ -494,14 +499,15 @@ int ensure_pipeline(ctbb_ctx *c, size_t rebin_bytes) {
 }
 
 // Item `item` of a pipeline (block b of its scan): prepare on ctx->stream into
-// filtered-projection buffer item % 2 (after the backprojection of item - 2
-// has read it), then backproject block b into d_image on stream_bp once the
-// preparation is done.
+// filtered-projection buffer item % kPipeDepth (after the backprojection of
+// item - kPipeDepth has read it), then backproject block b into d_image on
+// stream_bp once the preparation is done.
 int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int item, int b, const float *win, const BpView *d_vtab,
                    float *d_image) {
-    const int k = item & 1;
-    DevBuf &out = k ? c->rebin_out2 : c->rebin_out;
-    if (item >= 2 && hipStreamWaitEvent(c->stream, c->ev_bp[k], 0) != hipSuccess) return CTBB_ERR_HIP;
+    const int k = item % ctbb_ctx::kPipeDepth;
+    DevBuf &out = c->filt[k];
+    if (item >= ctbb_ctx::kPipeDepth && hipStreamWaitEvent(c->stream, c->ev_bp[k], 0) != hipSuccess)
+        return CTBB_ERR_HIP;
     int st = prep_block(c, mr, win, out.p);
     if (st) return st;
     if (hipEventRecord(c->ev_filt[k], c->stream) != hipSuccess ||
@@ -555,15 +561,17 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     hipStreamSynchronize(c->stream);
     if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
     if (c->stream_bp) hipStreamSynchronize(c->stream_bp);
-    for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->rebin_out, &c->image_block, &c->image_stack,
-                      &c->af_work, &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1], &c->rebin_out2, &c->image_stack2})
+    for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->image_block, &c->image_stack, &c->af_work,
+                      &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1],
+                      &c->image_stack2})
         b->release();
+    for (DevBuf &b : c->filt) b.release();
     for (int i = 0; i < 2; i++) {
         if (c->ev_ready[i]) hipEventDestroy(c->ev_ready[i]);
         if (c->ev_free[i]) hipEventDestroy(c->ev_free[i]);
     }
     if (c->copy_stream) hipStreamDestroy(c->copy_stream);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < ctbb_ctx::kPipeDepth; i++) {
         if (c->ev_filt[i]) hipEventDestroy(c->ev_filt[i]);
         if (c->ev_bp[i]) hipEventDestroy(c->ev_bp[i]);
     }

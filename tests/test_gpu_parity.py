@@ -315,6 +315,48 @@ def test_end_to_end_hu(ctx, tmp_path, mini_dir, ffs, table_dir):
     cb.free_metadata(mr)
 
 
+@pytest.mark.parametrize("ffs", ["none", "z", "phi", "both"])
+def test_pipeline_matches_stage_chain(ctx, tmp_path, mini_dir, ffs):
+    """The whole-scan pipeline (block b + 1 prepared on one stream while block
+    b is backprojected on another, two filtered-projection buffers) gives
+    exactly the result of the stage entry points called block by block
+    (adaptive filter, rebin + ramp filter, backprojection, finalize)."""
+    tok = scans.mini_tokens(ffs, nrows=8, nx=32, rotations=24, n_slices=40)
+    mr = _mr(tmp_path, tok, mini_dir)
+    frame = mr.cg.n_rows_raw * mr.cg.n_channels
+    n_read = tok["Readings"]
+    raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda")
+    ctx.synth_scan_dev(mr, scans.phantom_for(tok, 4), 0, n_read, raw)
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
+    s = cb.block_sizes(mr)
+    out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.reconstruct_dev(mr, raw, n_read, out)
+    got = host(out)
+    n_blocks = int(mr.ri.n_blocks)
+    assert n_blocks >= 3
+    stack = torch.empty(n_blocks * s.image_block, dtype=torch.float32, device="cuda")
+    changed = 0
+    for b in range(n_blocks):
+        cb.plan_block(mr, b)
+        sb = cb.block_sizes(mr)
+        s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+        assert s0 >= 0 and s0 + n <= n_read
+        win = raw[s0 * frame:(s0 + n) * frame]
+        af = torch.empty_like(win)
+        ctx.adaptive_filter_dev(mr, win, af)
+        changed += int((af != win).sum())
+        reb = torch.empty(sb.rebin, dtype=torch.float32, device="cuda")
+        ctx.rebin_filter_dev(mr, af, reb)
+        ctx.backproject_dev(mr, reb, stack[b * sb.image_block:(b + 1) * sb.image_block])
+    want = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.finalize_dev(mr, stack, want)
+    want = host(want)
+    print(f"pipeline vs stages {ffs}: {changed} samples smoothed, mismatches {np.count_nonzero(got != want)}")
+    assert changed > 0
+    assert np.array_equal(got, want)
+    cb.free_metadata(mr)
+
+
 @pytest.mark.parametrize("ffs", ["none", "both"])
 def test_reconstruct_host_matches_dev(ctx, tmp_path, mini_dir, ffs):
     """Host-streaming reconstruction (pull double buffer, overlap reuse, zero
