@@ -346,8 +346,14 @@ __device__ __forceinline__ void fft16(float2 (&v)[16]) {
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
 
 // Forward 4096-point DFT.  On entry v[r] = x[j + 256 r]; on exit
-// v[r] = X[j + 256 r].  tw: e^(-2 pi i t / 4096), t in [0, 4096).
+// v[r] = X[j + 256 r].  tw: e^(-2 pi i t / 4096), t in [0, 4096).  The 15
+// twiddles of each pass are requested one pass ahead, before the LDS
+// exchange, so their latency overlaps the butterflies and the barriers
+// instead of being paid once per twiddle.
 __device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, const float2 *__restrict__ tw) {
+    float2 w[15];
+#pragma unroll
+    for (int r = 1; r < 16; r++) w[r - 1] = tw[(16 * (j & 15) * r) & (FFT_L - 1)];
     fft16(v);                                           // pass Ns = 1: no twiddles
     __syncthreads();                                    // earlier readers of sh are done
 #pragma unroll
@@ -356,7 +362,9 @@ __device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, cons
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 256 * r)];
 #pragma unroll
-    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], tw[(16 * (j & 15) * r) & (FFT_L - 1)]);
+    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], w[r - 1]);
+#pragma unroll
+    for (int r = 1; r < 16; r++) w[r - 1] = tw[j * r];
     fft16(v);                                           // pass Ns = 16
     __syncthreads();
     const int base = (j >> 4) * 256 + (j & 15);
@@ -366,7 +374,7 @@ __device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, cons
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 256 * r)];
 #pragma unroll
-    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], tw[j * r]);
+    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], w[r - 1]);
     fft16(v);                                           // pass Ns = 256
 }
 
