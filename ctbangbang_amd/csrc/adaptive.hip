@@ -10,7 +10,8 @@
 //   1. view maximum         one workgroup per view, tree max (order-free)
 //   2. 18-degree smoothing  one thread per view, sequential fmaf sum
 //   3. 180-degree min/max + eccentricity   one thread per view
-//   4. threshold bisection  one wavefront per view, exact integer counts
+//   4. threshold bisection  one wavefront per view, exact integer counts over
+//      the view's window copied once to LDS
 //   5. 5x5x5 triangle smoothing of samples above threshold: one thread per
 //      sample (coalesced along channels)
 #include <hip/hip_runtime.h>
@@ -94,35 +95,52 @@ __global__ __launch_bounds__(256) void af_ecc(const float *sm, float *mx, float 
     ecc[p] = e > 1 ? 1.0f : (e < 0 ? 0.0f : e);
 }
 
-// rhs of reference preprocessing.cuh:32-62 evaluated by one wavefront
-__device__ float af_rhs_wave(const float *array, int start_idx, int max_idx, int numel_range, float T,
-                             float inv_pi_npf_a, float inv_npf) {
-    int diff = start_idx + (numel_range - 1);
+// The counting window of reference preprocessing.cuh:32-62 (clipped to the
+// pull), fixed for a view while the bisection moves T.
+__device__ __forceinline__ void af_window(int &start_idx, int &numel_range, int max_idx) {
+    const int diff = start_idx + (numel_range - 1);
     if (diff < numel_range - 1) {
         numel_range += start_idx;
         start_idx = 0;
     }
     if (diff > max_idx) numel_range -= (diff - max_idx);
+}
+
+// rhs of reference preprocessing.cuh:32-62 for a window of numel values:
+// the exact count of values above T, then the reference's float products.
+__device__ __forceinline__ float af_rhs_wave(const float *win, int numel, float T, float inv_pi_npf_a,
+                                             float inv_npf) {
     int cnt = 0;
-    for (int i = threadIdx.x & 63; i < numel_range; i += 64) cnt += array[start_idx + i] > T ? 1 : 0;
+    for (int i = threadIdx.x & 63; i < numel; i += 64) cnt += win[i] > T ? 1 : 0;
     for (int s = 32; s > 0; s >>= 1) cnt += __shfl_xor(cnt, s, 64);
     return inv_pi_npf_a * static_cast<float>(cnt) * inv_npf;
 }
 
-// reference preprocessing.cuh:167-206; one wavefront per view
-__global__ __launch_bounds__(256) void af_thresh(const float *sm, const float *mx, const float *mn,
-                                                 const float *ecc, float *thr, int n_proj, int n_range,
-                                                 int n_proj_ffs, float strength) {
-    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (p >= n_proj) return;
-    const int off = n_range / 2;
+// reference preprocessing.cuh:167-206; one wavefront per view.  The view's
+// counting window is copied to LDS once (kLds, when it fits in the dynamic
+// LDS the host gives the launch) and every bisection step counts from there;
+// otherwise each step reads it from memory.
+template <bool kLds>
+__global__ __launch_bounds__(64) void af_thresh(const float *sm, const float *mx, const float *mn, const float *ecc,
+                                                float *thr, int n_proj, int n_range, int n_proj_ffs,
+                                                float strength) {
+    extern __shared__ float win_lds[];
+    const int p = blockIdx.x;
     const int max_idx = n_proj - 1;
+    int start = p - n_range / 2, numel = n_range;
+    af_window(start, numel, max_idx);
+    const float *win = sm + start;
+    if (kLds) {
+        for (int i = threadIdx.x; i < numel; i += 64) win_lds[i] = win[i];
+        __syncthreads();
+        win = win_lds;
+    }
     const float inv_pi = 1.0f / 3.1415368979f;
     const float inv_npf = 1.0f / static_cast<float>(n_proj_ffs);
     float lower = mn[p], upper = mx[p];
     float T = (mx[p] + mn[p]) / 2.0f;
     const float lhs = 0.03f * strength * ecc[p];
-    float rhs = af_rhs_wave(sm, p - off, max_idx, n_range, T, inv_pi, inv_npf);
+    float rhs = af_rhs_wave(win, numel, T, inv_pi, inv_npf);
     int iter = 0;
     while (static_cast<double>(fabsf(lhs - rhs) / lhs) > 0.01) {
         if (iter >= 20) break;
@@ -131,9 +149,9 @@ __global__ __launch_bounds__(256) void af_thresh(const float *sm, const float *m
         else if (rhs < lhs) upper = T;
         else break;
         T = (lower + upper) / 2.0f;
-        rhs = af_rhs_wave(sm, p - off, max_idx, n_range, T, inv_pi, inv_npf);
+        rhs = af_rhs_wave(win, numel, T, inv_pi, inv_npf);
     }
-    if ((threadIdx.x & 63) == 0) thr[p] = T;
+    if (threadIdx.x == 0) thr[p] = T;
 }
 
 // reference preprocessing.cuh:208-285 (3D branch, __FILTER_3D__ defined)
@@ -190,8 +208,14 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
     hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
     hipLaunchKernelGGL(af_ecc, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, a.n_proj, n_range);
-    hipLaunchKernelGGL(af_thresh, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
-                       n_range, a.n_proj_ffs, a.strength);
+    // the counting window (at most n_range values) in LDS when it takes <= 16 KB
+    const size_t win_bytes = static_cast<size_t>(n_range) * sizeof(float);
+    if (win_bytes <= 16384)
+        hipLaunchKernelGGL(af_thresh<true>, dim3(a.n_proj), dim3(64), win_bytes, stream, sm, mx, mn, ecc, thr,
+                           a.n_proj, n_range, a.n_proj_ffs, a.strength);
+    else
+        hipLaunchKernelGGL(af_thresh<false>, dim3(a.n_proj), dim3(64), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
+                           n_range, a.n_proj_ffs, a.strength);
     hipLaunchKernelGGL(af_apply, dim3((a.nch + 255) / 256, a.n_rows_raw, a.n_proj), dim3(256), 0, stream, a.raw,
                        thr, a.out, a.n_proj, a.n_rows_raw, a.nch);
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
