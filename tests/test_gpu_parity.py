@@ -190,6 +190,40 @@ def test_backproject_random_sinogram(ctx, tmp_path, mini_dir, table_dir, nz):
 
 
 @pytest.mark.parametrize("nz", [32, 16])
+def test_backproject_tall_detector(ctx, tmp_path, mini_dir, nz):
+    """128 rebinned rows (z-FFS over 64 detector rows) at a field of view
+    whose footprint needs 8 channels per staging thread: with 16-slice
+    z-groups the staged window takes 76 KB of LDS per workgroup, above the
+    64 KB default, so the launch raises the limit first.  Random sinogram vs
+    the FP64 oracle: all but at most 1e-4 of the voxels within 2e-4 of the
+    image maximum.  The exceptions are the |q| = 1 steps of the weighting (a
+    view that is an angle's only contributor, at |q| within FP32 rounding of
+    1, adds its whole sample or nothing); with 64-128 rows a handful of the
+    524 288 voxels hit one.  The FP32 build of the oracle shows the same
+    steps against the FP64 one (8 voxels here, up to 0.079 of the maximum,
+    two of them the voxels where the HIP result steps)."""
+    import oracle
+    tok = scans.mini_tokens("z", nrows=128, nx=128)
+    mr = _mr(tmp_path, tok, mini_dir)
+    cb.plan_block(mr, 0)
+    assert mr.cg.n_rows == 128
+    s = cb.block_sizes(mr)
+    rng = np.random.default_rng(6)
+    reb = rng.standard_normal((s.n_proj_out, mr.cg.n_rows, mr.cg.n_channels_oversampled)).astype(np.float32)
+    img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
+    ctx.set_options(proj_dtype=cb.PROJ_F32, bp_tile=nz)
+    ctx.backproject_dev(mr, dev(reb), img)
+    got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
+    ctx.set_options(bp_tile=0)
+    want = oracle.backproject(mr, reb, oracle.BP_GPU, double=True)
+    err = np.abs(got - want) / np.abs(want).max()
+    n_off = int((err > 2e-4).sum())
+    print(f"bp tall detector (nz={nz}): rel max err {err.max():.3g}, {n_off} of {err.size} voxels beyond 2e-4")
+    assert n_off <= 1e-4 * err.size
+    cb.free_metadata(mr)
+
+
+@pytest.mark.parametrize("nz", [32, 16])
 def test_backproject_definitionas_block_sampled(ctx, tmp_path, nz):
     """Full-size phi+z FFS block (512^2, 64 rows): HIP vs FP64 oracle on 400
     sampled voxel columns plus the four 24x24 corner patches (voxels that
