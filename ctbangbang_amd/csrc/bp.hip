@@ -114,8 +114,8 @@ template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
 __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 1 ? 3 : 4))) void bp_kernel(
     BpArgs a) {
     constexpr int THREADS = 256 * ZS;
-    // dynamic LDS: 2 window buffers x [rows_pad + 1][RS] pairs, then a ring
-    // of 128 view-table entries
+    // dynamic LDS: 2 window buffers x [rows_pad + 1][RS] pairs, a ring of
+    // 128 view-table entries, and (two z-groups) 2 x 256 column geometries
     extern __shared__ float2 lds2[];
     const int tid = threadIdx.x;
 
@@ -219,6 +219,35 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     auto load_chunk = [&](int c) {
         if (tid < 64) ring[(c & 1) * 64 + tid] = vtab[min(64 * c + tid, n_views - 1)];
     };
+
+    // Per-column geometry of a view (window base plo), shared by the slices:
+    // channel fraction fp, LDS column index ci of the bilinear tap (within
+    // a window buffer), g = -td / (lhat tan(theta_cone/2)) and the z offset
+    // dz of slice 0 of the block, so that q(z) = g (dz + z0 coll + z coll) is
+    // qhat of reference include/backproject.cuh:84.
+    auto geometry = [&](const float4 v, int plo, float &fp, unsigned &ci, float &g, float &dz) {
+        const float phat = x * v.x - y * v.y;
+        const float pl = phat * a.inv_ds + a.two_cc - static_cast<float>(plo);
+        int c0 = static_cast<int>(floorf(pl));
+        fp = pl - static_cast<float>(c0);
+        c0 = min(max(c0, 0), W - 2);
+        ci = static_cast<unsigned>(c0) + static_cast<unsigned>(c0) / CPT;
+        const float xr = phat * a.inv_rf;
+        float asn = asin_small(xr);
+        if (!a.asin_small_only && __builtin_amdgcn_ballot_w64(fabsf(xr) > 0.45f))
+            asn = fabsf(xr) > 0.45f ? asinf(xr) : asn;
+        dz = fmaf(a.k_asin, asn, v.z);
+        // hardware square root (1 ulp): no denormal scaling or correction steps
+        const float lhat = __builtin_amdgcn_sqrtf(a.rf2 - phat * phat) - x * v.y - y * v.x;
+        g = a.neg_td * __builtin_amdgcn_rcpf(lhat * a.tanc);
+    };
+    // With two z-groups the groups take turns: during view t, group
+    // (t + 1) % 2 evaluates view t + 1 for its 256 columns and leaves it in
+    // LDS slot (t + 1) % 2 (published by the next barrier), so each
+    // evaluation serves both groups.
+    float4 *geo = ring + 128;
+    const int zg = tid >> 8;
+
     load_chunk(0);
     __syncthreads();
     int plo_next;
@@ -226,6 +255,12 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         const float4 v0 = ring[0];
         plo_next = window_lo(v0);
         fetch(__float_as_int(v0.w), plo_next);
+        if (ZS == 2 && zg == 0) {
+            float fp, g, dz;
+            unsigned ci;
+            geometry(v0, plo_next, fp, ci, g, dz);
+            geo[col] = make_float4(fp, __uint_as_float(ci), g, dz);
+        }
     }
 
     int k = 0;                                          // half turn of view t
@@ -246,36 +281,36 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         }
         __syncthreads();
         if ((t & 63) == 0 && t + 64 < n_views) load_chunk((t >> 6) + 1);
+        float fp, g, dz;
+        unsigned ci;
+        if (ZS == 2) {
+            const float4 gv = geo[(t & 1) * 256 + col];
+            fp = gv.x;
+            ci = __float_as_uint(gv.y);
+            g = gv.z;
+            dz = gv.w;
+        }
         const float4 vw = ring[t & 127];
         if (t + 1 < n_views) {
             const float4 vn = ring[(t + 1) & 127];
             plo_next = window_lo(vn);
             fetch(__float_as_int(vn.w), plo_next);
+            if (ZS == 2 && zg == ((t + 1) & 1)) {
+                float fpn, gn, dzn;
+                unsigned cin;
+                geometry(vn, plo_next, fpn, cin, gn, dzn);
+                geo[((t + 1) & 1) * 256 + col] = make_float4(fpn, __uint_as_float(cin), gn, dzn);
+            }
         }
+        if (ZS == 1) geometry(vw, p_lo, fp, ci, g, dz);
 
-        // per-column geometry, shared by the NZ slices:
-        // q(z) = A z + B is qhat of reference include/backproject.cuh:84
-        const float phat = x * vw.x - y * vw.y;
-        const float pl = phat * a.inv_ds + a.two_cc - static_cast<float>(p_lo);
-        int c0 = static_cast<int>(floorf(pl));
-        const float fp = pl - static_cast<float>(c0);
-        c0 = min(max(c0, 0), W - 2);
-        const float xr = phat * a.inv_rf;
-        float asn = asin_small(xr);
-        if (!a.asin_small_only && __builtin_amdgcn_ballot_w64(fabsf(xr) > 0.45f))
-            asn = fabsf(xr) > 0.45f ? asinf(xr) : asn;
-        const float dz0 = vw.z + a.k_asin * asn + static_cast<float>(z0) * a.coll;
-        // hardware square root (1 ulp): no denormal scaling or correction steps
-        const float lhat = __builtin_amdgcn_sqrtf(a.rf2 - phat * phat) - x * vw.y - y * vw.x;
-        const float g = a.neg_td * __builtin_amdgcn_rcpf(lhat * a.tanc);
         const float A = g * a.coll;
-        const float B = g * dz0;
+        const float B = g * fmaf(static_cast<float>(z0), a.coll, dz);
         // row coordinate (q + 1) hr = Ar z + Br, used directly where |q| < 0.6
         const float Ar = A * a.hr;
         const float Br = fmaf(B, a.hr, a.hr);
         // LDS index of (row 0, channel c0) in this view's buffer
-        const unsigned colb = static_cast<unsigned>((t & 1) * n_stage) + static_cast<unsigned>(c0) +
-                              static_cast<unsigned>(c0) / CPT;
+        const unsigned colb = static_cast<unsigned>((t & 1) * n_stage) + ci;
 
         // Slices in chunks of 4.  q is linear in z, so over a chunk it stays
         // within hw = 1.5 |A| of its value qm at the chunk centre: a chunk can
@@ -366,7 +401,7 @@ template <int NZ, int CPT, int TPR, int ZS>
 int launch_c(const BpArgs &a, hipStream_t stream) {
     const int rows_pad = (256 * ZS) >> a.tpr_log2;
     const size_t lds_bytes = 2 * static_cast<size_t>(rows_pad + 1) * a.tpr * (CPT + 1) * sizeof(float2) +
-                             128 * sizeof(float4);
+                             128 * sizeof(float4) + (ZS == 2 ? 2 * 256 * sizeof(float4) : 0);
     auto kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, true>)
                            : reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, false>);
     if (lds_bytes > 65536) {
