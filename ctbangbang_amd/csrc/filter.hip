@@ -345,15 +345,40 @@ __device__ __forceinline__ void fft16(float2 (&v)[16]) {
 
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
 
+// v[r] *= w^r for r = 1 .. 15, from b[m] = w^(2^m), m = 0..3: the other
+// powers are products of at most three of them (each product rounds once,
+// so the twiddles stay within a few FP32 ulps of the table values).
+__device__ __forceinline__ void twiddle16(float2 (&v)[16], const float2 (&b)[4]) {
+    const float2 w3 = cmul(b[1], b[0]);
+    const float2 w5 = cmul(b[2], b[0]);
+    const float2 w6 = cmul(b[2], b[1]);
+    const float2 w7 = cmul(b[2], w3);
+    v[1] = cmul(v[1], b[0]);
+    v[2] = cmul(v[2], b[1]);
+    v[3] = cmul(v[3], w3);
+    v[4] = cmul(v[4], b[2]);
+    v[5] = cmul(v[5], w5);
+    v[6] = cmul(v[6], w6);
+    v[7] = cmul(v[7], w7);
+    v[8] = cmul(v[8], b[3]);
+    v[9] = cmul(v[9], cmul(b[3], b[0]));
+    v[10] = cmul(v[10], cmul(b[3], b[1]));
+    v[11] = cmul(v[11], cmul(b[3], w3));
+    v[12] = cmul(v[12], cmul(b[3], b[2]));
+    v[13] = cmul(v[13], cmul(b[3], w5));
+    v[14] = cmul(v[14], cmul(b[3], w6));
+    v[15] = cmul(v[15], cmul(b[3], w7));
+}
+
 // Forward 4096-point DFT.  On entry v[r] = x[j + 256 r]; on exit
-// v[r] = X[j + 256 r].  tw: e^(-2 pi i t / 4096), t in [0, 4096).  The 15
-// twiddles of each pass are requested one pass ahead, before the LDS
-// exchange, so their latency overlaps the butterflies and the barriers
-// instead of being paid once per twiddle.
+// v[r] = X[j + 256 r].  tw: e^(-2 pi i t / 4096), t in [0, 4096).  Each
+// pass's twiddles are w^r of one w per thread; its four power-of-two powers
+// are requested one pass ahead (before the LDS exchange) and the other
+// eleven are formed by products.
 __device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, const float2 *__restrict__ tw) {
-    float2 w[15];
+    float2 b[4];
 #pragma unroll
-    for (int r = 1; r < 16; r++) w[r - 1] = tw[(16 * (j & 15) * r) & (FFT_L - 1)];
+    for (int m = 0; m < 4; m++) b[m] = tw[(16 << m) * (j & 15)];
     fft16(v);                                           // pass Ns = 1: no twiddles
     __syncthreads();                                    // earlier readers of sh are done
 #pragma unroll
@@ -361,10 +386,9 @@ __device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, cons
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 256 * r)];
+    twiddle16(v, b);
 #pragma unroll
-    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], w[r - 1]);
-#pragma unroll
-    for (int r = 1; r < 16; r++) w[r - 1] = tw[j * r];
+    for (int m = 0; m < 4; m++) b[m] = tw[j << m];
     fft16(v);                                           // pass Ns = 16
     __syncthreads();
     const int base = (j >> 4) * 256 + (j & 15);
@@ -373,8 +397,7 @@ __device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, cons
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 256 * r)];
-#pragma unroll
-    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], w[r - 1]);
+    twiddle16(v, b);
     fft16(v);                                           // pass Ns = 256
 }
 
