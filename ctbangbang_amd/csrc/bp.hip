@@ -172,7 +172,8 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         p_lo = __builtin_amdgcn_readfirstlane(p_lo);
         const int g0 = p_lo + scol;
         const size_t rb = (static_cast<size_t>(v) * a.n_rows + srow) * a.nos;
-        if (srow_ok && p_lo >= 0 && p_lo + W < a.nos) {
+        // (bfloat16 reads whole dwords, up to 3 samples past the window)
+        if (srow_ok && p_lo >= 0 && p_lo + W + (kBf16 ? 3 : 0) < a.nos) {
             // window inside the detector: CPT + 1 consecutive samples, read
             // through a buffer descriptor of projection v (raw buffer, gfx9
             // dword 3): per-lane offset toff_b, scalar offset p_lo, and the
@@ -183,16 +184,37 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<char *>(pb), static_cast<short>(0), a.n_rows * a.nos * static_cast<int>(kEs), 0x00020000);
             const int soff = p_lo * static_cast<int>(kEs);
+            if (kBf16) {
+                // bfloat16: whole dwords from the dword-aligned start of the
+                // lane's samples (vector and scalar offsets both even in
+                // samples), then each sample pair re-aligned by 0 or 16 bits;
+                // sub-dword loads are slower
+                constexpr int ND = CPT / 2 + 2;
+                const int lo_el = static_cast<int>(toff_b >> 1) + (p_lo & 1);   // past the even part of p_lo
+                const int par = lo_el & 1;
+                const int dbase = 2 * (lo_el - par);
+                const int soff2 = 2 * (p_lo & ~1);
+                uint32_t d[ND];
 #pragma unroll
-            for (int e = 0; e <= CPT; e++) {
-                if (kBf16)
-                    pv[e] = __builtin_amdgcn_raw_buffer_load_b16(rs, static_cast<int>(toff_b) + e * 2, soff, 0);
-                else
+                for (int q = 0; q < ND; q++) {
+                    d[q] = __builtin_amdgcn_raw_buffer_load_b32(rs, dbase + 4 * q, soff2, 0);
+                    asm volatile("" ::: "memory");
+                }
+#pragma unroll
+                for (int e = 0; e <= CPT; e += 2) {
+                    const uint32_t pr = __builtin_amdgcn_alignbit(d[e / 2 + 1], d[e / 2], 16u * par);
+                    pv[e] = pr & 0xffffu;
+                    if (e + 1 <= CPT) pv[e + 1] = pr >> 16;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e <= CPT; e++) {
                     pv[e] = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(toff_b) + e * 4, soff, 0);
-                // keep the loads separate: merged multi-dword loads land in
-                // register tuples that the (value, delta) pairs cannot use,
-                // and the copies out of them would wait for the loads here
-                asm volatile("" ::: "memory");
+                    // keep the loads separate: merged multi-dword loads land in
+                    // register tuples that the (value, delta) pairs cannot use,
+                    // and the copies out of them would wait for the loads here
+                    asm volatile("" ::: "memory");
+                }
             }
         } else {
 #pragma unroll

@@ -10,15 +10,15 @@
 //
 // Variants (ctbb_filter_mode):
 //   F32     v_mfma_f32_16x16x4_f32, FP32 operands, FP32 accumulation
-//   BF16X3  v_mfma_f32_16x16x16_bf16 on hi/lo bfloat16 splits of both
+//   BF16X3  v_mfma_f32_32x32x16_bf16 on hi/lo bfloat16 splits of both
 //           operands: X*T ~= Xh*Th + Xh*Tl + Xl*Th, FP32 accumulation
 //   F16X3   the same with float16 splits; taps are pre-scaled by 2^10 (and
 //           the result by 2^-10) to keep the low parts out of the subnormals
 //   FFT32   FP32 4096-point FFT convolution (the form of the reference's CPU
 //           path), for n <= 2048; wider rows fall back to F32
-// Tile: 64 rows x 64 columns per workgroup (4 waves, 2x2 of 32x32 per wave),
-// K chunks of 32 staged in LDS.  Requires n % 64 == 0 (1472 and 1344 for the
-// 736- and 672-channel scanners); other widths use a VALU kernel.
+// Toeplitz tiles are 64 (F32) or 256 (split forms) rows x 64 columns; they
+// require n % 64 == 0 (1472 and 1344 for the 736- and 672-channel scanners);
+// other widths use a VALU kernel.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -30,7 +30,6 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short i16x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64, BN = 64, BK = 32;
 constexpr int A_STRIDE = BK + 4;   // 36 floats: 16-byte aligned rows
@@ -125,123 +124,149 @@ __global__ __launch_bounds__(256) void toeplitz_f32_kernel(FilterArgs a) {
 }
 
 // Split-precision Toeplitz product: X = Xh + Xl and T = Th + Tl in bfloat16
-// (MODE BF16X3) or float16 (MODE F16X3, taps scaled by 2^10), accumulated
-// in FP32 as Xl*Th + Xh*Tl + Xh*Th on v_mfma_f32_16x16x16_{bf16,f16}.
-// A tile: split once into hi/lo 16-bit arrays in LDS (row stride 40).
-// B fragments: lane (col, k..k+3) needs T[n+col-k-e] = g[n-1-col+k+e] with
-// g the reversed taps; four copies of the reversed window shifted by 0..3
-// make every fragment one 8-byte-aligned LDS read.
+// (MODE BF16X3) or float16 (MODE F16X3, taps scaled by 2^10 to keep the low
+// parts out of the subnormals), accumulated in FP32 as Xl*Th + Xh*Tl + Xh*Th
+// on gfx950's v_mfma_f32_32x32x16_{bf16,f16} (measured on MI355X at about
+// 1.7x the rate of the 16x16x32 forms).  Workgroup = 256 rows x 64 columns;
+// each wave owns 64 rows x 64 columns as 2 x 2 tiles of 32 x 32.
+// The A operand goes straight from global memory into MFMA fragments: K is
+// reduced in chunks of 32, and within a chunk lane l (half h = l / 32) holds
+// k = 16 h + 8 s + e, e = 0..7, for k-step s, so its two A fragments are the
+// 16 consecutive samples of its row (four float4 loads), split into hi/lo
+// in registers and used for both column tiles; the next chunk's loads are
+// issued before the current chunk's MFMAs.  Only the reversed taps live in
+// LDS (four copies shifted by 0..3 so that every B fragment, 8 consecutive
+// reversed taps, is two 8-byte reads), written once, so the K loop has no
+// barriers.  D layout (32 x 32): lane l, register r -> row 8 (r / 4) + 4 h +
+// r % 4, column l % 32.  Workgroups are ordered so that the n / 64 column
+// tiles of one 256-row band run on the same XCD and share its L2 copy of the
+// band.
 template <int MODE>
-__global__ __launch_bounds__(256) void toeplitz_split_kernel(FilterArgs a) {
-    constexpr int AS = BK + 8;                 // A row stride in 16-bit elements
+__global__ __launch_bounds__(256) void toeplitz_split_kernel(FilterArgs a, int n_rt, int n_ct) {
+    constexpr int TM = 256, TN = 64, TK = 32;
+    typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+    typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+    typedef short i16x8 __attribute__((ext_vector_type(8)));
+    typedef float f32x16 __attribute__((ext_vector_type(16)));
     extern __shared__ uint16_t sh[];
     const int n = a.n;
-    const int tw = n + 72;                     // window length per copy
-    uint16_t *Ah = sh, *Al = sh + BM * AS;
-    uint16_t *Gh = Al + BM * AS;               // [4][tw]
+    const int tw = n + TN + 8;                 // window length per copy
+    uint16_t *Gh = sh;                         // [4][tw]
     uint16_t *Gl = Gh + 4 * tw;                // [4][tw]
+
+    int tile = blockIdx.x;
+    const int n_tiles = n_rt * n_ct;
+    if ((n_tiles & 7) == 0) tile = (blockIdx.x & 7) * (n_tiles >> 3) + (blockIdx.x >> 3);
+    const int rt_i = tile / n_ct, ct_i = tile - rt_i * n_ct;
+    const size_t m0 = static_cast<size_t>(rt_i) * TM;
+    const int n0 = ct_i * TN;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int lr = lane & 15, lk = lane >> 4;
-    const size_t m0 = static_cast<size_t>(blockIdx.y) * BM;
-    const int n0 = blockIdx.x * BN;
-    const int jb = n - 1 - (n0 + BN - 1);      // first reversed-tap index this tile needs
+    const int l32 = lane & 31, h = lane >> 5;
+    const int jb = n - 1 - (n0 + TN - 1);      // first reversed-tap index this tile needs
     const float tscale = MODE == CTBB_FILTER_F16X3 ? 1024.0f : 1.0f;
 
-    auto split = [&](float v, uint16_t &h, uint16_t &l) {
+    auto split = [&](float v, uint16_t &hi, uint16_t &lo) {
         if (MODE == CTBB_FILTER_BF16X3) {
             const __bf16 hb = static_cast<__bf16>(v);
-            h = __builtin_bit_cast(uint16_t, hb);
-            l = __builtin_bit_cast(uint16_t, static_cast<__bf16>(v - static_cast<float>(hb)));
+            hi = __builtin_bit_cast(uint16_t, hb);
+            lo = __builtin_bit_cast(uint16_t, static_cast<__bf16>(v - static_cast<float>(hb)));
         } else {
             const _Float16 hf = static_cast<_Float16>(v);
-            h = __builtin_bit_cast(uint16_t, hf);
-            l = __builtin_bit_cast(uint16_t, static_cast<_Float16>(v - static_cast<float>(hf)));
+            hi = __builtin_bit_cast(uint16_t, hf);
+            lo = __builtin_bit_cast(uint16_t, static_cast<_Float16>(v - static_cast<float>(hf)));
         }
     };
 
     for (int e = tid; e < 4 * tw; e += 256) {
-        const int s = e / tw, m = e - s * tw;
-        const int j = jb + m + s;             // reversed-tap index
+        const int sft = e / tw, m = e - sft * tw;
+        const int j = jb + m + sft;           // reversed-tap index
         const float t = (j >= 0 && j < 2 * n) ? a.taps[2 * n - 1 - j] * tscale : 0.0f;
         split(t, Gh[e], Gl[e]);
     }
+    __syncthreads();
 
-    f32x4 acc[2][2];
+    f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
 
-    const int arow = tid >> 2, aseg = (tid & 3) * 8;
-    const size_t gm = m0 + arow;
-    const bool row_ok = gm < a.rows;
-    const float *xrow = a.in + gm * n + aseg;
-    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
-    if (row_ok) {
-        x0 = reinterpret_cast<const float4 *>(xrow)[0];
-        x1 = reinterpret_cast<const float4 *>(xrow)[1];
+    // fragment rows of this lane; rows past the end read the last row (their
+    // results are not stored)
+    const float *xrow[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++) {
+        size_t gm = m0 + wave * 64 + rt * 32 + l32;
+        if (gm >= a.rows) gm = a.rows - 1;
+        xrow[rt] = a.in + gm * n + 16 * h;
     }
-
-    for (int k0 = 0; k0 < n; k0 += BK) {
-        __syncthreads();   // previous tile consumed (and taps written, first pass)
-        {
-            const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-            uint16_t h[8], l[8];
+    auto load = [&](float4 (&x)[2][4], int k0) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) split(xs[e], h[e], l[e]);
-            i16x4 h0 = {(short)h[0], (short)h[1], (short)h[2], (short)h[3]};
-            i16x4 h1 = {(short)h[4], (short)h[5], (short)h[6], (short)h[7]};
-            i16x4 l0 = {(short)l[0], (short)l[1], (short)l[2], (short)l[3]};
-            i16x4 l1 = {(short)l[4], (short)l[5], (short)l[6], (short)l[7]};
-            i16x4 *ph = reinterpret_cast<i16x4 *>(Ah + arow * AS + aseg);
-            i16x4 *pl = reinterpret_cast<i16x4 *>(Al + arow * AS + aseg);
-            ph[0] = h0; ph[1] = h1;
-            pl[0] = l0; pl[1] = l1;
-        }
-        __syncthreads();
-        // prefetch the next K chunk of this thread's row while the MFMAs run
-        if (k0 + BK < n && row_ok) {
-            x0 = reinterpret_cast<const float4 *>(xrow + k0 + BK)[0];
-            x1 = reinterpret_cast<const float4 *>(xrow + k0 + BK)[1];
-        }
+        for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            i16x4 ah[2], al[2], bh[2], bl[2];
+            for (int q = 0; q < 4; q++) x[rt][q] = *reinterpret_cast<const float4 *>(xrow[rt] + k0 + 4 * q);
+    };
+    auto mma = [&](const float4 (&x)[2][4], int k0) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            i16x8 ah[2], al[2];
 #pragma unroll
             for (int rt = 0; rt < 2; rt++) {
-                const int row = wr * 32 + rt * 16 + lr;
-                ah[rt] = *reinterpret_cast<const i16x4 *>(Ah + row * AS + kk + 4 * lk);
-                al[rt] = *reinterpret_cast<const i16x4 *>(Al + row * AS + kk + 4 * lk);
+                const float4 u = x[rt][2 * s], w = x[rt][2 * s + 1];
+                const float v[8] = {u.x, u.y, u.z, u.w, w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    uint16_t hi, lo;
+                    split(v[e], hi, lo);
+                    ah[rt][e] = static_cast<short>(hi);
+                    al[rt][e] = static_cast<short>(lo);
+                }
             }
 #pragma unroll
             for (int ct = 0; ct < 2; ct++) {
-                const int col = n0 + wc * 32 + ct * 16 + lr;
-                const int mm = (n - 1 - col + k0 + kk + 4 * lk) - jb;
+                const int col = n0 + ct * 32 + l32;
+                const int mm = (n - 1 - col + k0 + 16 * h + 8 * s) - jb;
                 const int sft = mm & 3;
-                bh[ct] = *reinterpret_cast<const i16x4 *>(Gh + sft * tw + (mm - sft));
-                bl[ct] = *reinterpret_cast<const i16x4 *>(Gl + sft * tw + (mm - sft));
-            }
+                const uint16_t *ph = Gh + sft * tw + (mm - sft);
+                const uint16_t *pl = Gl + sft * tw + (mm - sft);
+                const i16x4 h0 = *reinterpret_cast<const i16x4 *>(ph), h1 = *reinterpret_cast<const i16x4 *>(ph + 4);
+                const i16x4 l0 = *reinterpret_cast<const i16x4 *>(pl), l1 = *reinterpret_cast<const i16x4 *>(pl + 4);
+                const i16x8 bh = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
+                const i16x8 bl = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
 #pragma unroll
-            for (int rt = 0; rt < 2; rt++)
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++) {
+                for (int rt = 0; rt < 2; rt++) {
                     if (MODE == CTBB_FILTER_BF16X3) {
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rt], bl[ct], acc[rt][ct], 0, 0, 0);
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rt], bh[ct], acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, al[rt]), __builtin_bit_cast(bf16x8, bh), acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, ah[rt]), __builtin_bit_cast(bf16x8, bl), acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, ah[rt]), __builtin_bit_cast(bf16x8, bh), acc[rt][ct], 0, 0, 0);
                     } else {
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
-                            __builtin_bit_cast(f16x4, al[rt]), __builtin_bit_cast(f16x4, bh[ct]), acc[rt][ct], 0, 0, 0);
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
-                            __builtin_bit_cast(f16x4, ah[rt]), __builtin_bit_cast(f16x4, bl[ct]), acc[rt][ct], 0, 0, 0);
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16f16(
-                            __builtin_bit_cast(f16x4, ah[rt]), __builtin_bit_cast(f16x4, bh[ct]), acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            __builtin_bit_cast(f16x8, al[rt]), __builtin_bit_cast(f16x8, bh), acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            __builtin_bit_cast(f16x8, ah[rt]), __builtin_bit_cast(f16x8, bl), acc[rt][ct], 0, 0, 0);
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            __builtin_bit_cast(f16x8, ah[rt]), __builtin_bit_cast(f16x8, bh), acc[rt][ct], 0, 0, 0);
                     }
                 }
+            }
         }
+    };
+
+    // n % 64 == 0, so the chunks come in pairs
+    float4 xa[2][4], xb[2][4];
+    load(xa, 0);
+    for (int k0 = 0; k0 < n; k0 += 2 * TK) {
+        load(xb, k0 + TK);
+        mma(xa, k0);
+        if (k0 + 2 * TK < n) load(xa, k0 + 2 * TK);
+        mma(xb, k0 + TK);
     }
 
     const float oscale = 1.0f / tscale;
@@ -250,11 +275,11 @@ __global__ __launch_bounds__(256) void toeplitz_split_kernel(FilterArgs a) {
 #pragma unroll
         for (int ct = 0; ct < 2; ct++)
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const size_t om = m0 + wr * 32 + rt * 16 + 4 * lk + e;
-                const int col = n0 + wc * 32 + ct * 16 + lr;
+            for (int r = 0; r < 16; r++) {
+                const size_t om = m0 + wave * 64 + rt * 32 + 8 * (r / 4) + 4 * h + (r % 4);
+                const int col = n0 + ct * 32 + l32;
                 if (om < a.rows) {
-                    const float y = acc[rt][ct][e] * oscale;
+                    const float y = acc[rt][ct][r] * oscale;
                     if (a.out_bf16)
                         reinterpret_cast<uint16_t *>(a.out)[om * n + col] = f32_to_bf16_rne(y);
                     else
@@ -477,10 +502,37 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
         hipLaunchKernelGGL(conv_valu_kernel, dim3(static_cast<unsigned>(a.rows)), dim3(256), lds, stream, a);
         return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
     }
+    if (a.mode == CTBB_FILTER_BF16X3 || a.mode == CTBB_FILTER_F16X3) {
+        // 256 x 64 tiles on the double-K MFMAs; > 64 KB of LDS per workgroup
+        constexpr int TM = 256;
+        const size_t n_rt = (a.rows + TM - 1) / TM, n_ct = static_cast<size_t>(a.n) / 64;
+        if (n_rt * n_ct > 0x7fffffffu) return CTBB_ERR_SHAPE;
+        const size_t lds2 = 8 * (static_cast<size_t>(a.n) + 72) * sizeof(uint16_t);
+        const void *kf = a.mode == CTBB_FILTER_BF16X3
+                             ? reinterpret_cast<const void *>(&toeplitz_split_kernel<CTBB_FILTER_BF16X3>)
+                             : reinterpret_cast<const void *>(&toeplitz_split_kernel<CTBB_FILTER_F16X3>);
+        if (lds2 > 65536) {
+            int dev = 0, max_lds = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+                return CTBB_ERR_HIP;
+            if (lds2 > static_cast<size_t>(max_lds)) return CTBB_ERR_SHAPE;
+            if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds2)) !=
+                hipSuccess)
+                return CTBB_ERR_SHAPE;
+        }
+        const dim3 grid(static_cast<unsigned>(n_rt * n_ct));
+        if (a.mode == CTBB_FILTER_BF16X3)
+            hipLaunchKernelGGL((toeplitz_split_kernel<CTBB_FILTER_BF16X3>), grid, dim3(256), lds2, stream, a,
+                               static_cast<int>(n_rt), static_cast<int>(n_ct));
+        else
+            hipLaunchKernelGGL((toeplitz_split_kernel<CTBB_FILTER_F16X3>), grid, dim3(256), lds2, stream, a,
+                               static_cast<int>(n_rt), static_cast<int>(n_ct));
+        return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
+    }
     const size_t gy = (a.rows + BM - 1) / BM;
-    const size_t lds = (a.mode == CTBB_FILTER_F32 || a.mode == CTBB_FILTER_FFT32)
-                           ? BM * A_STRIDE * sizeof(float) + 2 * a.n * sizeof(float)
-                           : (2 * BM * (BK + 8) + 8 * (static_cast<size_t>(a.n) + 72)) * sizeof(uint16_t);
+    // FP32 Toeplitz product (CTBB_FILTER_F32, and FFT32 rows wider than 2048)
+    const size_t lds = BM * A_STRIDE * sizeof(float) + 2 * a.n * sizeof(float);
     if (lds > 64 * 1024) return CTBB_ERR_SHAPE;
     // grid.y may exceed 65535 for very tall inputs: split into launches
     const size_t max_y = 65535;
@@ -492,13 +544,8 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
         else s.out = reinterpret_cast<float *>(a.out) + y0 * BM * a.n;
         s.rows = a.rows - y0 * BM < ny * BM ? a.rows - y0 * BM : ny * BM;
         dim3 grid(a.n / BN, static_cast<unsigned>(ny));
-        switch (a.mode) {
-            case CTBB_FILTER_F32:
-            case CTBB_FILTER_FFT32: hipLaunchKernelGGL(toeplitz_f32_kernel, grid, dim3(256), lds, stream, s); break;
-            case CTBB_FILTER_BF16X3: hipLaunchKernelGGL(toeplitz_split_kernel<CTBB_FILTER_BF16X3>, grid, dim3(256), lds, stream, s); break;
-            case CTBB_FILTER_F16X3: hipLaunchKernelGGL(toeplitz_split_kernel<CTBB_FILTER_F16X3>, grid, dim3(256), lds, stream, s); break;
-            default: return CTBB_ERR_ARG;
-        }
+        if (a.mode != CTBB_FILTER_F32 && a.mode != CTBB_FILTER_FFT32) return CTBB_ERR_ARG;
+        hipLaunchKernelGGL(toeplitz_f32_kernel, grid, dim3(256), lds, stream, s);
         if (hipGetLastError() != hipSuccess) return CTBB_ERR_HIP;
     }
     return CTBB_OK;
