@@ -37,9 +37,10 @@ def parse():
                     help="skip the PCIe-inclusive run (scan in page-locked host memory, final stack to host)")
     ap.add_argument("--cpu-columns", type=int, default=1024, help="voxel columns in the CPU baseline sample")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_traffic_c1.json"))
-    ap.add_argument("--stage", default="all", choices=["all", "backproject", "copy"],
-                    help="all: whole volume (the benchmark); backproject: only the backprojection of one block "
-                         "(for counter runs); copy: a 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--stage", default="all", choices=["all", "backproject", "filter", "copy"],
+                    help="all: whole volume (the benchmark); backproject / filter: only the backprojection / the "
+                         "ramp filter (--filter form) of one block (for counter runs); copy: a 1 GiB device copy "
+                         "(HBM counter calibration)")
     ap.add_argument("--bp-tile", type=int, default=16, choices=[16, 32])
     ap.add_argument("--filter", default=None, choices=["fft32", "f32", "bf16x3", "f16x3"],
                     help="ramp filter form (default: fft32 for --form plain, f16x3 for --form mixed)")
@@ -90,6 +91,36 @@ def stage_backproject(args, cb, scans, mr, ctx, raw, frame):
                       "updates_per_s": s.voxel_updates_block / avg,
                       "gather_GB_per_s": BP_BYTES_PER_UPDATE * s.voxel_updates_block / avg / 1e9,
                       "bp_tile": args.bp_tile, "form": args.form}), flush=True)
+
+
+def stage_filter(args, cb, mr, ctx, raw, frame, filt):
+    """Ramp filter of the middle block's rebinned rows only, K launches: the
+    FFT form or a Toeplitz product on the matrix cores (--filter)."""
+    import torch
+    cb.plan_block(mr, int(mr.ri.n_blocks) // 2)
+    s = cb.block_sizes(mr)
+    s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+    win = raw[s0 * frame:(s0 + n) * frame]
+    reb = torch.empty(s.rebin, dtype=torch.float32, device="cuda:0")
+    ctx.rebin_dev(mr, win, reb)
+    nos = int(mr.cg.n_channels_oversampled)
+    rows = s.rebin // nos
+    taps = torch.from_numpy(cb.load_filter(mr)).to("cuda:0")
+    out = torch.empty_like(reb)
+    for _ in range(args.warmup):
+        ctx.filter_rows_dev(taps, nos, rows, reb, out)
+    ctx.synchronize()
+    total = 0.0
+    for _ in range(args.steps):
+        ctx.filter_rows_dev(taps, nos, rows, reb, out)
+        total += ctx.kernel_time("filter")[0]
+    avg_ms = total / args.steps
+    line = {"stage": "filter", "filter": filt, "rows": rows, "n": nos, "avg_launch_ms": avg_ms,
+            "GB_per_s_in_out": 2.0 * 4 * rows * nos / avg_ms / 1e6}
+    if filt != "fft32":
+        products = 3 if filt in ("bf16x3", "f16x3") else 1
+        line["matrix_TFLOP_per_s"] = products * 2.0 * rows * nos * nos / avg_ms / 1e9
+    print(json.dumps(line), flush=True)
 
 
 def bench_c3(args):
@@ -226,6 +257,10 @@ def main():
     ctx.synth_scan_dev(mr, scans.phantom_for(tok, {"c0": 0, "c1": 1, "c4": 4}[args.config]), 0, n_read, raw)
     if args.stage == "backproject":
         stage_backproject(args, cb, scans, mr, ctx, raw, frame)
+        ctx.close()
+        return
+    if args.stage == "filter":
+        stage_filter(args, cb, mr, ctx, raw, frame, filt)
         ctx.close()
         return
     s = cb.block_sizes(mr)
