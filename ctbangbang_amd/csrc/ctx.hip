@@ -291,7 +291,6 @@ int fill_bp_args(const recon_metadata *mr, BpArgs &a) {
     a.tanc = static_cast<float>(std::tan(static_cast<double>(cg.theta_cone) / 2.0));
     a.neg_td = static_cast<float>(-cg.table_direction);
     a.hr = (static_cast<float>(cg.n_rows) - 1.0f) / 2.0f;
-    a.row_max = static_cast<float>(cg.n_rows) - 1.0001f;
     const double dmax = dx > dy ? dx : dy;
     a.half_w = static_cast<float>(7.5 * std::sqrt(2.0) * dmax / ds + 1.0);
     a.acq_half = rp.acq_fov / 2.0f;
@@ -596,6 +595,8 @@ int ctbb_ctx_set_options(ctbb_ctx *c, const ctbb_options *o) {
     if (!c || !o) return CTBB_ERR_ARG;
     if (o->filter_mode < CTBB_FILTER_F32 || o->filter_mode > CTBB_FILTER_FFT32) return CTBB_ERR_ARG;
     if (o->proj_dtype != CTBB_PROJ_F32 && o->proj_dtype != CTBB_PROJ_BF16) return CTBB_ERR_ARG;
+    if (o->bp_tile != 0 && o->bp_tile != 16 && o->bp_tile != 32) return CTBB_ERR_ARG;
+    if (o->bp_views_per_pass != 0 && o->bp_views_per_pass != 1) return CTBB_ERR_ARG;
     c->opt = *o;
     return CTBB_OK;
 }

@@ -27,7 +27,7 @@ struct BpArgs {
     int n_tiles;
     float xs, xc, x0;          // x(xi) = xs * (xi - xc) + x0
     float ys, yc, y0;          // y(yi) = ys * (yi - yc) + y0
-    float inv_ds, two_cc, inv_rf, rf2, k_asin, coll, tanc, neg_td, hr, row_max;
+    float inv_ds, two_cc, inv_rf, rf2, k_asin, coll, tanc, neg_td, hr;
     float half_w;              // footprint half width of a 16x16 tile, channels
     float acq_half, out_scale;
 };

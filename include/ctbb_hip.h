@@ -136,9 +136,9 @@ typedef struct ctbb_options_s {
     int filter_mode;            /* ctbb_filter_mode; default CTBB_FILTER_FFT32 */
     int proj_dtype;             /* ctbb_proj_dtype */
     float mu_water;             /* >0: convert the final stack to HU */
-    int bp_tile;                /* backprojection slices per thread: 16 (two per workgroup, the
-                                   default, 0) or 32 */
-    int bp_views_per_pass;      /* angles staged per LDS pass (0 = default) */
+    int bp_tile;                /* backprojection slices per thread: 16 (two z-groups per
+                                   workgroup; 0 selects it) or 32; other values are rejected */
+    int bp_views_per_pass;      /* reserved: the kernels stage one view per LDS pass (0 or 1) */
     int adaptive_filter;        /* 1 = run the adaptive raw-data filter (default) */
 } ctbb_options;
 

@@ -298,3 +298,25 @@ def test_binary_reader_roundtrip(tmp_path, mini_dir):
     want[:(hi - s) * frame] = data[s * frame:hi * frame]
     assert np.array_equal(got, want)
     cb.free_metadata(mr)
+
+
+def test_batching_workload_plans(tmp_path):
+    """All 32 seeded scans of the batching workload plan with the reference
+    arithmetic: every pull lies inside the scan's readings, at most 400
+    slices of 1 mm, and the stated total of 7.544e12 voxel-updates."""
+    from ctbangbang_amd import scans
+    total = 0.0
+    for i in range(32):
+        tok = scans.c3_tokens(i)
+        p = str(tmp_path / f"c3_{i}.prm")
+        cb.write_prm(p, tok)
+        mr = cb.metadata_from_prm(p)
+        assert 1 <= int(tok["NSlices"]) <= 400
+        assert mr.rp.nx in (512, 768)
+        for b in range(mr.ri.n_blocks):
+            cb.plan_block(mr, b)
+            assert mr.ri.idx_pull_start >= 0
+            assert mr.ri.idx_pull_start + mr.ri.n_proj_pull <= int(tok["Readings"])
+            total += cb.block_sizes(mr).voxel_updates_block
+        cb.free_metadata(mr)
+    assert abs(total - 7.544e12) / 7.544e12 < 1e-3

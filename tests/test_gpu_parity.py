@@ -518,3 +518,16 @@ def test_reference_host_contract(ctx, tmp_path, mini_dir):
     assert np.array_equal(got, want)
     mr.ctd.final_image_stack = None
     cb.free_metadata(mr)
+
+
+def test_context_options_validated(ctx):
+    """Unsupported option values are rejected and leave the options unchanged."""
+    before = ctx.options()
+    for bad in (dict(bp_tile=7), dict(bp_views_per_pass=2), dict(filter_mode=9), dict(proj_dtype=5)):
+        with pytest.raises(cb.CTBBError):
+            ctx.set_options(**bad)
+        now = ctx.options()
+        assert (now.bp_tile, now.bp_views_per_pass, now.filter_mode, now.proj_dtype) == \
+            (before.bp_tile, before.bp_views_per_pass, before.filter_mode, before.proj_dtype)
+    ctx.set_options(bp_tile=32)
+    ctx.set_options(bp_tile=0)
