@@ -520,6 +520,43 @@ def test_reference_host_contract(ctx, tmp_path, mini_dir):
     cb.free_metadata(mr)
 
 
+def test_driver_executable_matches_dev(ctx, tmp_path, mini_dir):
+    """The reference-style driver (main.cu:69-271 over the compat stage
+    functions) run as a program on a prm file writes the same stack as the
+    device-resident pipeline with the driver's options."""
+    import subprocess
+    tok = scans.mini_tokens("phi", nrows=8, nx=32)
+    tok["RawDataDir"] = str(tmp_path)
+    tok["OutputDir"] = str(tmp_path)
+    tok["OutputFile"] = "driver.img"
+    mr = _mr(tmp_path, tok, mini_dir)
+    pp = scans.phantom_for(tok, 4)
+    n_read = tok["Readings"]
+    raw_h = cb.synth_scan(mr, pp, 0, n_read)
+    raw_h.tofile(tmp_path / "synthetic.raw")
+    p = str(tmp_path / "driver.prm")
+    cb.write_prm(p, tok)
+    env = dict(os.environ, CTBB_INSTALL_DIR=mini_dir)
+    exe = os.path.join(cb.PKG_DIR, "ctbangbang")
+    print(f"driver: {exe} --device=0 {p}", flush=True)
+    r = subprocess.run([exe, "--device=0", "--timing", p], capture_output=True, text=True, env=env, timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ms for entire reconstruction" in r.stdout
+    got = np.fromfile(tmp_path / "driver.img", dtype=np.float32)
+    # the driver's options: F32 filter, FP32 projections, adaptive filter,
+    # HU with the prm's MuWater
+    ctx.set_options(mu_water=float(tok["MuWater"]), adaptive_filter=1, filter_mode=cb.FILTER_F32,
+                    proj_dtype=cb.PROJ_F32)
+    s = cb.block_sizes(mr)
+    out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.reconstruct_dev(mr, dev(raw_h.ravel()), n_read, out)
+    want = host(out)
+    ctx.set_options(mu_water=0.0, filter_mode=cb.FILTER_FFT32)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+    cb.free_metadata(mr)
+
+
 def test_context_options_validated(ctx):
     """Unsupported option values are rejected and leave the options unchanged."""
     before = ctx.options()
