@@ -37,10 +37,10 @@ def parse():
                     help="skip the PCIe-inclusive run (scan in page-locked host memory, final stack to host)")
     ap.add_argument("--cpu-columns", type=int, default=1024, help="voxel columns in the CPU baseline sample")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_traffic_c1.json"))
-    ap.add_argument("--stage", default="all", choices=["all", "backproject", "filter", "copy"],
+    ap.add_argument("--stage", default="all", choices=["all", "backproject", "filter", "hu16", "copy"],
                     help="all: whole volume (the benchmark); backproject / filter: only the backprojection / the "
-                         "ramp filter (--filter form) of one block (for counter runs); copy: a 1 GiB device copy "
-                         "(HBM counter calibration)")
+                         "ramp filter (--filter form) of one block (for counter runs); hu16: the int16 output "
+                         "conversion of the final stack; copy: a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--bp-tile", type=int, default=16, choices=[16, 32])
     ap.add_argument("--filter", default=None, choices=["fft32", "f32", "bf16x3", "f16x3"],
                     help="ramp filter form (default: fft32 for --form plain, f16x3 for --form mixed)")
@@ -121,6 +121,29 @@ def stage_filter(args, cb, mr, ctx, raw, frame, filt):
         products = 3 if filt in ("bf16x3", "f16x3") else 1
         line["matrix_TFLOP_per_s"] = products * 2.0 * rows * nos * nos / avg_ms / 1e9
     print(json.dumps(line), flush=True)
+
+
+def stage_hu16(args, cb, mr, ctx):
+    """Output conversion of the final stack (HU float32 -> int16 stored values
+    for a DICOM series), K launches over a synthetic HU stack of the final
+    stack's size: HBM-bound, 6 bytes per voxel."""
+    import torch
+    s = cb.block_sizes(mr)
+    n = int(s.final_stack)
+    g = torch.Generator(device="cuda:0").manual_seed(5)
+    hu = torch.randn(n, generator=g, device="cuda:0") * 1000.0
+    out = torch.empty(n, dtype=torch.int16, device="cuda:0")
+    for _ in range(args.warmup):
+        ctx.hu_to_int16_dev(hu, n, out)
+    ctx.synchronize()
+    total = 0.0
+    for _ in range(args.steps):
+        ctx.hu_to_int16_dev(hu, n, out)
+        total += ctx.kernel_time("hu16")[0]
+    avg_ms = total / args.steps
+    gbs = 6.0 * n / avg_ms / 1e6
+    print(json.dumps({"stage": "hu16", "voxels": n, "avg_launch_ms": avg_ms, "GB_per_s": gbs,
+                      "hbm_frac": gbs / 8000.0}), flush=True)
 
 
 def bench_c3(args):
@@ -251,6 +274,10 @@ def main():
                     mu_water=float(tok.get("MuWater", 0.0)),
                     adaptive_filter=1 if float(tok["AdaptiveFiltration"]) > 0 else 0,
                     bp_tile=args.bp_tile)
+    if args.stage == "hu16":
+        stage_hu16(args, cb, mr, ctx)
+        ctx.close()
+        return
     n_read = int(tok["Readings"])
     frame = mr.cg.n_rows_raw * mr.cg.n_channels
     raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda:0")

@@ -94,6 +94,8 @@ _SIGS = {
     "ctbb_filter_rows_dev": (C.c_int, [_VP, _VP, C.c_int, C.c_size_t, _VP, _VP]),
     "ctbb_backproject_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_finalize_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
+    "ctbb_hu_to_int16_dev": (C.c_int, [_VP, _VP, C.c_size_t, C.c_float, C.c_float, _VP]),
+    "ctbb_write_dicom_series": (C.c_int, [_MR, _VP, C.c_float, C.c_float, C.c_char_p, C.c_char_p]),
     "ctbb_reconstruct_dev": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP, _VP]),
     "ctbb_reconstruct_host": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP]),
     "ctbb_reconstruct_batch_dev": (C.c_int, [_VP, C.c_int, C.POINTER(_MR), C.POINTER(_VP), C.POINTER(C.c_size_t),
@@ -182,6 +184,16 @@ def block_sizes(mr):
 
 def n_slices_final(mr):
     return int(lib().ctbb_n_slices_final(C.byref(mr)))
+
+
+def write_dicom_series(mr, stack_i16, directory, prefix="slice", slope=1.0, intercept=0.0):
+    """Final stack of int16 stored values ([n_slices_final][nx][ny], host
+    memory) as a DICOM CT series <directory>/<prefix>_NNNN.dcm."""
+    a = np.ascontiguousarray(stack_i16, dtype=np.int16)
+    if a.size != n_slices_final(mr) * mr.rp.nx * mr.rp.ny:
+        raise ValueError("stack size does not match the final stack of mr")
+    _check(lib().ctbb_write_dicom_series(C.byref(mr), a.ctypes.data, float(slope), float(intercept),
+                                         os.fsencode(directory), os.fsencode(prefix)), "write_dicom_series")
 
 
 def load_filter(mr):
@@ -295,6 +307,12 @@ class Context:
 
     def finalize_dev(self, mr, d_image, d_final):
         _check(lib().ctbb_finalize_dev(self._h, C.byref(mr), _ptr(d_image), _ptr(d_final)), "finalize_dev")
+
+    def hu_to_int16_dev(self, d_hu, n, d_out, slope=1.0, intercept=0.0):
+        """int16 stored values of a HU stack (HU = slope * stored + intercept):
+        round half to even, saturated, NaN -> 0."""
+        _check(lib().ctbb_hu_to_int16_dev(self._h, _ptr(d_hu), int(n), float(slope), float(intercept),
+                                          _ptr(d_out)), "hu_to_int16_dev")
 
     def reconstruct_dev(self, mr, d_raw, n_raw_readings, d_final, d_image=None):
         _check(lib().ctbb_reconstruct_dev(self._h, C.byref(mr), _ptr(d_raw), int(n_raw_readings), _ptr(d_image),

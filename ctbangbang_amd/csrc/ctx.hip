@@ -682,6 +682,17 @@ int ctbb_finalize_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float 
     return do_finalize(c, mr, d_image, d_final);
 }
 
+int ctbb_hu_to_int16_dev(ctbb_ctx *c, const float *d_hu, size_t n, float slope, float intercept, int16_t *d_out) {
+    if (!c || (n && (!d_hu || !d_out))) return CTBB_ERR_ARG;
+    hipSetDevice(c->device);
+    c->reset_events();
+    Hu16Args a{d_hu, d_out, n, slope, intercept};
+    EventPair e = c->begin("hu16");
+    const int st = launch_hu16(a, c->stream);
+    c->end(e);
+    return st;
+}
+
 int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_raw, size_t n_raw_readings,
                          float *d_image, float *d_final) {
     if (!c || !mr || !d_raw || !d_final || !mr->table_positions || !mr->tube_angles) return CTBB_ERR_ARG;

@@ -85,6 +85,15 @@ struct FinalizeArgs {
 };
 int launch_finalize(const FinalizeArgs &a, hipStream_t stream);
 
+// HU -> int16 stored pixel values (HU = slope * stored + intercept)
+struct Hu16Args {
+    const float *in;
+    int16_t *out;
+    size_t n;
+    float slope, intercept;
+};
+int launch_hu16(const Hu16Args &a, hipStream_t stream);
+
 // ---------------------------------------------------------------------------
 // Synthetic scan generator
 int launch_synth(const struct recon_metadata &mr, const ctbb_phantom_params &pp, long first, long count,

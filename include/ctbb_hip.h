@@ -154,7 +154,8 @@ void *ctbb_ctx_stream(ctbb_ctx *ctx);
 int ctbb_ctx_synchronize(ctbb_ctx *ctx);
 /* Device time of a kernel family during the most recent top-level call on
  * the context, from HIP events recorded on the context stream around each
- * launch.  family: "adaptive", "rebin", "filter", "backproject", "finalize".
+ * launch.  family: "adaptive", "rebin", "filter", "backproject", "finalize",
+ * "hu16".
  * Synchronises the stream; *total_ms = summed duration, *launches = count. */
 int ctbb_ctx_kernel_time(ctbb_ctx *ctx, const char *family, float *total_ms, int *launches);
 
@@ -223,6 +224,28 @@ int ctbb_backproject_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
 /* Reorder + thicken (+HU) d_image [n_slices_recon][nx][ny] into d_final. */
 int ctbb_finalize_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
                       const float *d_image, float *d_final);
+
+/* Output format: int16 stored values of a stack in HU, for a DICOM series
+ * (HU = slope * stored + intercept, the Rescale Slope / Intercept of the
+ * files): d_out[i] = (d_hu[i] - intercept) / slope in IEEE FP32, rounded half
+ * to even, saturated to [-32768, 32767], NaN -> 0.  slope > 0 and finite,
+ * intercept finite, else CTBB_ERR_ARG; n = 0 does nothing.  No counterpart in
+ * the reference, which writes the float32 stack only (src/setup.cu:769-787). */
+int ctbb_hu_to_int16_dev(ctbb_ctx *ctx, const float *d_hu, size_t n, float slope, float intercept,
+                         int16_t *d_out);
+
+/* Write a final stack of int16 stored values ([n_slices_final][nx][ny] host
+ * memory, as ctbb_hu_to_int16_dev produces from the HU stack) as a DICOM CT
+ * series: one Part-10 file per slice, explicit VR little endian, named
+ * <dir>/<prefix>_NNNN.dcm (NNNN = slice number from 1).  Geometry in the
+ * scanner frame of the reconstruction: slice k at z = start_pos +
+ * direction * k * slice_thickness, pixel (row i, column j) of a slice is
+ * voxel (x index i, y index j) of the stack, with the voxel centres of
+ * reference include/backproject.cuh:59-64.  Study, series and instance UIDs
+ * are new per call (2.25 root).  CTBB_ERR_IO if a file cannot be written.
+ * No counterpart in the reference. */
+int ctbb_write_dicom_series(const struct recon_metadata *mr, const int16_t *h_stack, float slope,
+                            float intercept, const char *dir, const char *prefix);
 
 /* Whole reconstruction from a device-resident scan: every block of the plan
  * in mr (ctbb_plan_reconstruction), then finalize, into d_final.  d_raw holds

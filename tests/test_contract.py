@@ -320,3 +320,73 @@ def test_batching_workload_plans(tmp_path):
             total += cb.block_sizes(mr).voxel_updates_block
         cb.free_metadata(mr)
     assert abs(total - 7.544e12) / 7.544e12 < 1e-3
+
+
+@pytest.mark.parametrize("table_dir,reverse", [("out", False), ("in", True)])
+def test_dicom_series_writer(tmp_path, mini_dir, table_dir, reverse):
+    """ctbb_write_dicom_series: Part-10 structure, explicit VR little endian
+    encoding rules, geometry of every slice and the pixel data round trip."""
+    import dicomfile as D
+    tok = scans.mini_tokens("none", nrows=8, nx=32, table_dir=table_dir, n_slices=5)
+    if reverse:
+        # descending slice positions (NSlices 0: the end position is used as given)
+        tok["StartPos"], tok["EndPos"] = tok["EndPos"], tok["StartPos"]
+        tok["NSlices"] = 0
+    p = str(tmp_path / "d.prm")
+    cb.write_prm(p, tok)
+    mr = cb.metadata_from_prm(p, install_dir=mini_dir)
+    nf, nx, ny = cb.n_slices_final(mr), mr.rp.nx, mr.rp.ny
+    assert nf in (4, 5)      # floor(|end - start| / thickness) + 1 in float32
+    rng = np.random.default_rng(3)
+    stack = rng.integers(-32768, 32768, size=(nf, nx, ny)).astype(np.int16)
+    stack[0, 0, :4] = [-32768, 32767, -1, 0]
+    out = tmp_path / "dcm"
+    out.mkdir()
+    cb.write_dicom_series(mr, stack, str(out), "ser", slope=0.5, intercept=-1024.0)
+    files = sorted(os.listdir(out))
+    assert files == [f"ser_{k + 1:04d}.dcm" for k in range(nf)]
+    direction = -1.0 if reverse else 1.0
+    assert (mr.rp.end_pos - mr.rp.start_pos) * direction > 0
+    fov, thick = mr.rp.recon_fov, mr.rp.slice_thickness
+    sx = 1.0 if mr.cg.table_direction == -1 else -1.0
+    series, insts = set(), set()
+    for k, name in enumerate(files):
+        meta, data, order = D.read(str(out / name))
+        assert order == sorted(order[:len(meta)]) + sorted(order[len(meta):])
+        # group length = bytes of the meta elements after it
+        glen = sum(8 + len(v) + (4 if vr in D.LONG_VRS else 0)
+                   for (g, e), (vr, v) in meta.items() if (g, e) != (2, 0))
+        assert meta[(2, 0)] == ("UL", glen.to_bytes(4, "little"))
+        assert D.text(meta[(2, 0x10)]) == "1.2.840.10008.1.2.1"
+        assert D.text(meta[(2, 2)]) == D.text(data[(8, 0x16)]) == "1.2.840.10008.5.1.4.1.1.2"
+        assert D.text(meta[(2, 3)]) == D.text(data[(8, 0x18)])
+        assert D.text(data[(8, 0x60)]) == "CT"
+        for tag in [(2, 3), (2, 0x12), (8, 0x18), (0x20, 0xD), (0x20, 0xE), (0x20, 0x52)]:
+            src = meta if tag[0] == 2 else data
+            assert D.valid_uid(D.text(src[tag])), tag
+        series.add(D.text(data[(0x20, 0xE)]))
+        insts.add(D.text(data[(8, 0x18)]))
+        assert int(D.text(data[(0x20, 0x13)])) == k + 1
+        rows, cols = D.us(data[(0x28, 0x10)]), D.us(data[(0x28, 0x11)])
+        assert (rows, cols) == (nx, ny)
+        assert [D.us(data[(0x28, t)]) for t in (2, 0x100, 0x101, 0x102, 0x103)] == [1, 16, 16, 15, 1]
+        assert D.text(data[(0x28, 4)]) == "MONOCHROME2"
+        np.testing.assert_allclose(D.ds(data[(0x28, 0x30)]), [fov / nx, fov / ny], rtol=1e-6)
+        assert D.ds(data[(0x28, 0x1052)]) == [-1024.0] and D.ds(data[(0x28, 0x1053)]) == [0.5]
+        z = mr.rp.start_pos + direction * k * thick
+        pos = D.ds(data[(0x20, 0x32)])
+        want = [sx * (fov / nx) * (-(nx - 1) / 2) + mr.rp.x_origin, (fov / ny) * (-(ny - 1) / 2) + mr.rp.y_origin, z]
+        np.testing.assert_allclose(pos, want, rtol=1e-6, atol=1e-4)
+        np.testing.assert_allclose(D.ds(data[(0x20, 0x1041)]), [z], atol=1e-4)
+        assert D.ds(data[(0x20, 0x37)]) == [0.0, 1.0, 0.0, sx, 0.0, 0.0]
+        np.testing.assert_allclose(D.ds(data[(0x18, 0x50)]), [thick], rtol=1e-6)
+        np.testing.assert_array_equal(D.pixels(data[(0x7FE0, 0x10)], rows, cols), stack[k])
+    assert len(series) == 1 and len(insts) == nf
+    # bad arguments
+    with pytest.raises(cb.CTBBError):
+        cb.write_dicom_series(mr, stack, str(out), "ser", slope=0.0)
+    with pytest.raises(cb.CTBBError):
+        cb.write_dicom_series(mr, stack, str(tmp_path / "missing_dir"), "ser")
+    with pytest.raises(ValueError):
+        cb.write_dicom_series(mr, stack[:-1], str(out), "ser")
+    cb.free_metadata(mr)

@@ -557,6 +557,69 @@ def test_driver_executable_matches_dev(ctx, tmp_path, mini_dir):
     cb.free_metadata(mr)
 
 
+def _hu16_ref(x, slope, intercept):
+    """int16 stored values in numpy float32: IEEE subtraction and division,
+    round half to even, NaN -> 0, saturation."""
+    v = np.rint((x.astype(np.float32) - np.float32(intercept)) / np.float32(slope))
+    v = np.where(np.isnan(v), np.float32(0), v)
+    return np.clip(v, -32768, 32767).astype(np.int16)
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 8, 9, 4099, 1_000_003])
+@pytest.mark.parametrize("offset", [0, 1])
+@pytest.mark.parametrize("slope,intercept", [(1.0, 0.0), (0.5, -1024.0), (2.5, 3.0)])
+def test_hu_to_int16_bit_exact(ctx, n, offset, slope, intercept):
+    rng = np.random.default_rng(n + 7 * offset)
+    x = (rng.standard_normal(n + 1) * 3000.0).astype(np.float32)
+    special = np.array([0.5, 1.5, 2.5, -0.5, -1.5, np.nan, np.inf, -np.inf, 1e9, -1e9, 32767.5, -32768.5,
+                        16383.75, -0.0], dtype=np.float32)
+    x[1:1 + min(n, special.size)] = special[:min(n, special.size)]
+    xd = dev(x)
+    out = torch.full((n + 2,), 12345, dtype=torch.int16, device="cuda")
+    # offset 1 element: neither buffer 16-byte aligned (single-sample path)
+    ctx.hu_to_int16_dev(xd[offset:offset + n], n, out[offset:offset + n], slope=slope, intercept=intercept)
+    got = host(out)
+    np.testing.assert_array_equal(got[offset:offset + n], _hu16_ref(x[offset:offset + n], slope, intercept))
+    assert (np.delete(got, np.arange(offset, offset + n)) == 12345).all()      # nothing written outside
+
+
+def test_hu_to_int16_rejects_bad_rescale(ctx):
+    x = torch.zeros(16, dtype=torch.float32, device="cuda")
+    o = torch.zeros(16, dtype=torch.int16, device="cuda")
+    for slope, intercept in [(0.0, 0.0), (-1.0, 0.0), (float("nan"), 0.0), (float("inf"), 0.0), (1.0, float("nan"))]:
+        with pytest.raises(cb.CTBBError):
+            ctx.hu_to_int16_dev(x, 16, o, slope=slope, intercept=intercept)
+
+
+def test_dicom_series_from_reconstruction(ctx, tmp_path, mini_dir):
+    """Reconstruction in HU -> int16 on the device -> DICOM series: the
+    pixel data read back equals the device's int16 stack, which is the HU
+    stack rounded."""
+    import dicomfile as D
+    tok = scans.mini_tokens("z", nrows=8, nx=32, n_slices=6)
+    mr = _mr(tmp_path, tok, mini_dir)
+    pp = scans.phantom_for(tok, 2)
+    n_read = tok["Readings"]
+    raw = dev(cb.synth_scan(mr, pp, 0, n_read).ravel())
+    ctx.set_options(mu_water=float(tok["MuWater"]), adaptive_filter=1, filter_mode=cb.FILTER_FFT32,
+                    proj_dtype=cb.PROJ_F32)
+    s = cb.block_sizes(mr)
+    hu = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.reconstruct_dev(mr, raw, n_read, hu)
+    i16 = torch.empty(s.final_stack, dtype=torch.int16, device="cuda")
+    ctx.hu_to_int16_dev(hu, s.final_stack, i16)
+    hu_h, st = host(hu), host(i16)
+    ctx.set_options(mu_water=0.0)
+    np.testing.assert_array_equal(st, _hu16_ref(hu_h, 1.0, 0.0))
+    assert np.abs(st - hu_h).max() <= 0.5
+    nf, nx, ny = cb.n_slices_final(mr), mr.rp.nx, mr.rp.ny
+    cb.write_dicom_series(mr, st.reshape(nf, nx, ny), str(tmp_path), "recon")
+    for k in range(nf):
+        _, data, _ = D.read(str(tmp_path / f"recon_{k + 1:04d}.dcm"))
+        np.testing.assert_array_equal(D.pixels(data[(0x7FE0, 0x10)], nx, ny), st.reshape(nf, nx, ny)[k])
+    cb.free_metadata(mr)
+
+
 def test_context_options_validated(ctx):
     """Unsupported option values are rejected and leave the options unchanged."""
     before = ctx.options()
