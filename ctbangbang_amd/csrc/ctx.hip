@@ -30,7 +30,7 @@ struct DevBuf {
     size_t bytes = 0;
     int ensure(size_t n) {
         if (n <= bytes) return CTBB_OK;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
         if (hipMalloc(&p, n) != hipSuccess) return CTBB_ERR_NOMEM;
@@ -38,7 +38,7 @@ struct DevBuf {
         return CTBB_OK;
     }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
@@ -75,26 +75,31 @@ struct ctbb_ctx {
     int map_kind = 0, map_sets = 0;
     std::map<std::string, std::vector<EventPair>> events;
     std::vector<EventPair> free_events;
+    bool timing_failed = false;
 
     EventPair begin(const char *family, hipStream_t on = nullptr) {
         EventPair e;
         if (!free_events.empty()) {
             e = free_events.back();
             free_events.pop_back();
-        } else {
-            hipEventCreate(&e.a);
-            hipEventCreate(&e.b);
+        } else if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) {
+            // no timing for this launch (ctbb_ctx_kernel_time reports an error)
+            timing_failed = true;
+            return EventPair{nullptr, nullptr, nullptr};
         }
         e.s = on ? on : stream;
-        hipEventRecord(e.a, e.s);
+        if (hipEventRecord(e.a, e.s) != hipSuccess) timing_failed = true;
         events[family].push_back(e);
         return e;
     }
-    void end(const EventPair &e) { hipEventRecord(e.b, e.s); }
+    void end(const EventPair &e) {
+        if (e.b && hipEventRecord(e.b, e.s) != hipSuccess) timing_failed = true;
+    }
     void reset_events() {
         for (auto &kv : events)
             for (auto &e : kv.second) free_events.push_back(e);
         events.clear();
+        timing_failed = false;
     }
 };
 
@@ -539,7 +544,7 @@ int ctbb_ctx_create(int device, ctbb_ctx **out) {
         if (hipSetDevice(device) != hipSuccess) return CTBB_ERR_HIP;
     }
     ctbb_ctx *c = new ctbb_ctx();
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return CTBB_ERR_HIP;
@@ -556,32 +561,32 @@ int ctbb_ctx_create(int device, ctbb_ctx **out) {
 
 void ctbb_ctx_destroy(ctbb_ctx *c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
-    if (c->stream_bp) hipStreamSynchronize(c->stream_bp);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    if (c->stream_bp) (void)hipStreamSynchronize(c->stream_bp);
     for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->image_block, &c->image_stack, &c->af_work,
                       &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1],
                       &c->image_stack2})
         b->release();
     for (DevBuf &b : c->filt) b.release();
     for (int i = 0; i < 2; i++) {
-        if (c->ev_ready[i]) hipEventDestroy(c->ev_ready[i]);
-        if (c->ev_free[i]) hipEventDestroy(c->ev_free[i]);
+        if (c->ev_ready[i]) (void)hipEventDestroy(c->ev_ready[i]);
+        if (c->ev_free[i]) (void)hipEventDestroy(c->ev_free[i]);
     }
-    if (c->copy_stream) hipStreamDestroy(c->copy_stream);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (int i = 0; i < ctbb_ctx::kPipeDepth; i++) {
-        if (c->ev_filt[i]) hipEventDestroy(c->ev_filt[i]);
-        if (c->ev_bp[i]) hipEventDestroy(c->ev_bp[i]);
+        if (c->ev_filt[i]) (void)hipEventDestroy(c->ev_filt[i]);
+        if (c->ev_bp[i]) (void)hipEventDestroy(c->ev_bp[i]);
     }
-    if (c->ev_sync) hipEventDestroy(c->ev_sync);
-    if (c->stream_bp) hipStreamDestroy(c->stream_bp);
+    if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
+    if (c->stream_bp) (void)hipStreamDestroy(c->stream_bp);
     c->reset_events();
     for (auto &e : c->free_events) {
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
-    hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -614,6 +619,7 @@ int ctbb_ctx_kernel_time(ctbb_ctx *c, const char *family, float *total_ms, int *
     if (ctbb_ctx_synchronize(c) != CTBB_OK) return CTBB_ERR_HIP;
     *total_ms = 0.0f;
     *launches = 0;
+    if (c->timing_failed) return CTBB_ERR_HIP;
     auto it = c->events.find(family);
     if (it == c->events.end()) return CTBB_OK;
     for (auto &e : it->second) {
@@ -629,7 +635,7 @@ int ctbb_adaptive_filter_dev(ctbb_ctx *c, const struct recon_metadata *mr, const
     if (!c || !mr || !d_raw || !d_out) return CTBB_ERR_ARG;
     int st = check_geometry(mr);
     if (st) return st;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     return do_adaptive(c, mr, d_raw, d_out);
 }
@@ -638,7 +644,7 @@ int ctbb_rebin_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_
     if (!c || !mr || !d_raw || !d_rebin) return CTBB_ERR_ARG;
     int st = check_geometry(mr);
     if (st) return st;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     return do_rebin(c, mr, d_raw, d_rebin);
 }
@@ -646,7 +652,7 @@ int ctbb_rebin_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_
 int ctbb_filter_rows_dev(ctbb_ctx *c, const float *d_taps, int n, size_t n_rows_total, const float *d_in,
                          void *d_out) {
     if (!c || !d_taps || !d_in || !d_out || n <= 0) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     if (c->opt.filter_mode == CTBB_FILTER_FFT32) {
         // the spectrum of these taps (the cached one belongs to a prm's filter)
@@ -663,28 +669,28 @@ int ctbb_filter_rows_dev(ctbb_ctx *c, const float *d_taps, int n, size_t n_rows_
 
 int ctbb_rebin_filter_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_raw, void *d_rebin) {
     if (!c || !mr || !d_raw || !d_rebin) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     return rebin_filter_dev(c, mr, d_raw, d_rebin);
 }
 
 int ctbb_backproject_dev(ctbb_ctx *c, const struct recon_metadata *mr, const void *d_rebin, float *d_image_block) {
     if (!c || !mr || !d_rebin || !d_image_block) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     return backproject_dev(c, mr, d_rebin, d_image_block);
 }
 
 int ctbb_finalize_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_image, float *d_final) {
     if (!c || !mr || !d_image || !d_final) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     return do_finalize(c, mr, d_image, d_final);
 }
 
 int ctbb_hu_to_int16_dev(ctbb_ctx *c, const float *d_hu, size_t n, float slope, float intercept, int16_t *d_out) {
     if (!c || (n && (!d_hu || !d_out))) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     Hu16Args a{d_hu, d_out, n, slope, intercept};
     EventPair e = c->begin("hu16");
@@ -696,7 +702,7 @@ int ctbb_hu_to_int16_dev(ctbb_ctx *c, const float *d_hu, size_t n, float slope, 
 int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_raw, size_t n_raw_readings,
                          float *d_image, float *d_final) {
     if (!c || !mr || !d_raw || !d_final || !mr->table_positions || !mr->tube_angles) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     const ct_geom &cg = mr->cg;
     const int n_blocks = static_cast<int>(mr->ri.n_blocks);
@@ -772,7 +778,7 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
 int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h_raw, size_t n_raw_readings,
                           float *h_final) {
     if (!c || !mr || !h_raw || !h_final || !mr->table_positions || !mr->tube_angles) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     const ct_geom &cg = mr->cg;
     const int n_blocks = static_cast<int>(mr->ri.n_blocks);
@@ -904,7 +910,7 @@ int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *
         if (!m || !d_raws[s] || !d_finals[s] || !m->table_positions || !m->tube_angles || m->ri.n_blocks <= 0)
             return CTBB_ERR_ARG;
     }
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     // every scan must use the filter table and rebin maps of the first one
     int st = ensure_taps(c, mrs[0]);
     if (st) return st;
@@ -1017,7 +1023,7 @@ int ctbb_adaptive_filter(ctbb_ctx *c, struct recon_metadata *mr) {
     if (!c || !mr || !mr->ctd.raw) return CTBB_ERR_ARG;
     int st = check_geometry(mr);
     if (st) return st;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     const size_t n = mr->ri.n_proj_pull * mr->cg.n_rows_raw * mr->cg.n_channels;
     if ((st = c->raw_in.ensure(n * sizeof(float))) || (st = c->raw_af.ensure(n * sizeof(float)))) return st;
@@ -1034,7 +1040,7 @@ int ctbb_rebin_filter(ctbb_ctx *c, struct recon_metadata *mr) {
     if (!c || !mr || !mr->ctd.raw) return CTBB_ERR_ARG;
     int st = check_geometry(mr);
     if (st) return st;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     const size_t n = mr->ri.n_proj_pull * mr->cg.n_rows_raw * mr->cg.n_channels;
     if ((st = c->raw_in.ensure(n * sizeof(float)))) return st;
@@ -1046,7 +1052,7 @@ int ctbb_rebin_filter(ctbb_ctx *c, struct recon_metadata *mr) {
     if (hipMalloc(&d, sz.rebin * proj_elem_bytes(c)) != hipSuccess) return CTBB_ERR_NOMEM;
     st = rebin_filter_dev(c, mr, c->raw_in.as<float>(), d);
     if (st) {
-        hipFree(d);
+        (void)hipFree(d);
         return st;
     }
     if (mr->flags.testing && mr->ctd.rebin && c->opt.proj_dtype == CTBB_PROJ_F32) {
@@ -1054,7 +1060,7 @@ int ctbb_rebin_filter(ctbb_ctx *c, struct recon_metadata *mr) {
             return CTBB_ERR_HIP;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
-        hipFree(d);
+        (void)hipFree(d);
         return CTBB_ERR_HIP;
     }
     mr->ctd.d_rebin = static_cast<float *>(d);
@@ -1063,7 +1069,7 @@ int ctbb_rebin_filter(ctbb_ctx *c, struct recon_metadata *mr) {
 
 int ctbb_backproject(ctbb_ctx *c, struct recon_metadata *mr) {
     if (!c || !mr || !mr->ctd.d_rebin || !mr->ctd.image || mr->ri.cb.block_idx < 1) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     const size_t nblk = mr->ri.n_slices_block * static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
     int st = c->image_block.ensure(nblk * sizeof(float));
@@ -1074,14 +1080,14 @@ int ctbb_backproject(ctbb_ctx *c, struct recon_metadata *mr) {
     if (hipMemcpyAsync(dst, c->image_block.p, nblk * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         return CTBB_ERR_HIP;
     st = hip_ok(hipStreamSynchronize(c->stream));
-    hipFree(mr->ctd.d_rebin);
+    (void)hipFree(mr->ctd.d_rebin);
     mr->ctd.d_rebin = nullptr;
     return st;
 }
 
 int ctbb_finalize_image_stack(ctbb_ctx *c, struct recon_metadata *mr) {
     if (!c || !mr || !mr->ctd.image) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
     const size_t n_img = mr->ri.n_slices_recon * nxy;
@@ -1112,7 +1118,7 @@ int ctbb_finalize_image_stack(ctbb_ctx *c, struct recon_metadata *mr) {
 int ctbb_synth_scan_dev(ctbb_ctx *c, const struct recon_metadata *mr, const ctbb_phantom_params *pp, long first,
                         long count, float *d_out) {
     if (!c || !mr || !pp || !d_out) return CTBB_ERR_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     return launch_synth(*mr, *pp, first, count, d_out, c->stream);
 }
 

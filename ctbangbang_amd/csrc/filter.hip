@@ -39,7 +39,6 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return static_cast<uint16_t>(u >> 16);
 }
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
 // FP32 Toeplitz product on v_mfma_f32_16x16x4_f32.
 __global__ __launch_bounds__(256) void toeplitz_f32_kernel(FilterArgs a) {

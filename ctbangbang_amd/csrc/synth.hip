@@ -50,7 +50,7 @@ int launch_synth(const recon_metadata &mr, const ctbb_phantom_params &pp, long f
         if (hipGetLastError() != hipSuccess) st = CTBB_ERR_HIP;
     }
     if (hipStreamSynchronize(stream) != hipSuccess) st = CTBB_ERR_HIP;
-    hipFree(dE);
+    (void)hipFree(dE);
     return st;
 }
 
