@@ -15,8 +15,8 @@ circular convolution of length M = 2*2^ceil(log2 N)).
 Pinning status (see oracle.c header): backproject and finalize are pinned
 against the reference's own CPU code; the filter against numpy.convolve and
 its closed-form taps; rebin by closed-form invariants; the adaptive filter by
-invariants only ("parity unpinned" against the reference itself, whose stage
-is GPU-only).
+closed forms and invariants of its own algorithm (the reference's stage is
+GPU-only CUDA, so no reference build exists to compare against).
 """
 import ctypes as C
 import os
