@@ -94,13 +94,16 @@ __device__ __forceinline__ float asin_small(float x) {
 // staged window covers rows_pad x W channels, rows_pad = 256 / TPR a power of
 // two >= n_rows, W = TPR * CPT: thread t stages row t / TPR, channels
 // (t % TPR) * CPT ... + CPT - 1 from CPT + 1 consecutive samples of that row.
-// LDS holds (value, delta) pairs row-major; every run of CPT pairs is
-// followed by one pad pair, so channel c of row r sits at r * RS + c + c / CPT
-// with RS = TPR * (CPT + 1): a wave's staging writes land at a stride of
-// CPT + 1 pairs, and its bilinear reads of neighbouring columns at
-// neighbouring channels.  Row rows_pad is kept zero so a row coordinate that
-// rounds to exactly n_rows - 1 reads a finite neighbour (its weight is 0).
+// LDS holds (value, delta) pairs row-major.  For even CPT every run of CPT
+// pairs is followed by one pad pair (PAD = 1), for odd CPT there is none, so
+// channel c of row r sits at r * RS + c + PAD * (c / CPT) with
+// RS = TPR * (CPT + PAD): a wave's staging writes land at an odd stride of
+// CPT + PAD pairs, 16 consecutive lanes on 16 different 8-byte bank pairs.
+// Row rows_pad is kept zero so a row coordinate that rounds to exactly
+// n_rows - 1 reads a finite neighbour (its weight is 0).
 static_assert(sizeof(BpView) == 16 && alignof(BpView) <= 16, "view table entries are read as float4");
+
+template <int CPT> constexpr int kPad = (CPT & 1) ? 0 : 1;
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
@@ -126,9 +129,13 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     }
     const int tx = tile / a.tiles_y, ty = tile - tx * a.tiles_y;
+    // Column of this thread in the 16x16 tile.  Each wave covers 4 x-rows of
+    // 16; within it each group of 16 lanes (one LDS cycle of a 64-bit read)
+    // takes a 4x4 patch, whose projections spread over fewer window channels
+    // than a 1x16 line, so their LDS reads meet fewer bank conflicts.
     const int col = tid & 255;
-    const int xi = tx * 16 + (col >> 4);
-    const int yi = ty * 16 + (col & 15);
+    const int xi = tx * 16 + 4 * (col >> 6) + ((col >> 2) & 3);
+    const int yi = ty * 16 + 4 * ((col >> 4) & 3) + (col & 3);
     const int z0 = (blockIdx.y * ZS + (tid >> 8)) * NZ;
 
     const float x = a.xs * (static_cast<float>(xi) - a.xc) + a.x0;
@@ -141,7 +148,8 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     const int tpr_log2 = TPR ? ilog2(TPR) : a.tpr_log2;
     const int rows_pad = THREADS >> tpr_log2;
     const int W = tpr * CPT;                            // staged channels
-    const int RS = tpr * (CPT + 1);                     // pairs per staged row
+    constexpr int PAD = kPad<CPT>;
+    const int RS = tpr * (CPT + PAD);                   // pairs per staged row
     const int n_stage = (rows_pad + 1) * RS;            // pairs per buffer
     const int srow = tid >> tpr_log2;                   // staged row of this thread
     const int scol = (tid & (tpr - 1)) * CPT;           // first staged channel
@@ -252,7 +260,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         int c0 = static_cast<int>(floorf(pl));
         fp = pl - static_cast<float>(c0);
         c0 = min(max(c0, 0), W - 2);
-        ci = static_cast<unsigned>(c0) + static_cast<unsigned>(c0) / CPT;
+        ci = static_cast<unsigned>(c0) + (PAD ? static_cast<unsigned>(c0) / CPT : 0u);
         const float xr = phat * a.inv_rf;
         float asn = asin_small(xr);
         if (!a.asin_small_only && __builtin_amdgcn_ballot_w64(fabsf(xr) > 0.45f))
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         const int p_lo = plo_next;
         float2 *buf = lds2 + (t & 1) * n_stage;
         {
-            float2 *dst = buf + tid * (CPT + 1);        // = srow * RS + scol + scol / CPT
+            float2 *dst = buf + tid * (CPT + PAD);      // = srow * RS + scol + PAD * (scol / CPT)
             float prev = bits_to_f32<kBf16>(pv[0]);
 #pragma unroll
             for (int e = 0; e < CPT; e++) {
@@ -421,7 +429,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
 template <int NZ, int CPT, int TPR, int ZS>
 int launch_c(const BpArgs &a, hipStream_t stream) {
     const int rows_pad = (256 * ZS) >> a.tpr_log2;
-    const size_t lds_bytes = 2 * static_cast<size_t>(rows_pad + 1) * a.tpr * (CPT + 1) * sizeof(float2) +
+    const size_t lds_bytes = 2 * static_cast<size_t>(rows_pad + 1) * a.tpr * (CPT + kPad<CPT>) * sizeof(float2) +
                              128 * sizeof(float4) + (ZS == 2 ? 2 * 256 * sizeof(float4) : 0);
     auto kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, true>)
                            : reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, false>);
