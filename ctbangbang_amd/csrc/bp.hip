@@ -136,7 +136,14 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     const int col = tid & 255;
     const int xi = tx * 16 + 4 * (col >> 6) + ((col >> 2) & 3);
     const int yi = ty * 16 + 4 * ((col >> 4) & 3) + (col & 3);
-    const int z0 = (blockIdx.y * ZS + (tid >> 8)) * NZ;
+    // Slices of this thread.  With two z-groups the groups interleave by
+    // chunks of 4 (group g: slices 8m + 4g .. 8m + 4g + 3 of the block's 32),
+    // so both see about the same number of visible chunks per view and
+    // neither waits long at the view's barrier; chunk zc of the thread is
+    // slice z0 + ZSTEP * zc + c.
+    static_assert(ZS == 1 || NZ * ZS == 32, "two z-groups split the block's 32 slices");
+    constexpr int ZSTEP = ZS == 2 ? 2 : 1;
+    const int z0 = ZS == 2 ? blockIdx.y * 32 + 4 * (tid >> 8) : blockIdx.y * NZ;
 
     const float x = a.xs * (static_cast<float>(xi) - a.xc) + a.x0;
     const float y = a.ys * (static_cast<float>(yi) - a.yc) + a.y0;
@@ -360,19 +367,19 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         const float two_hr = 2.0f * a.hr;
 #pragma unroll
         for (int zc = 0; zc < NZ; zc += 4) {
-            const float qm = fmaf(A, static_cast<float>(zc) + 1.5f, B);
+            const float qm = fmaf(A, static_cast<float>(ZSTEP * zc) + 1.5f, B);
             if (__builtin_amdgcn_ballot_w64(fabsf(qm) < t_vis) == 0) continue;
             float rowf[4], w[4];
             if (__builtin_amdgcn_ballot_w64(!(fabsf(qm) < t_int)) == 0) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    rowf[c] = fmaf(Ar, static_cast<float>(zc + c), Br);
+                    rowf[c] = fmaf(Ar, static_cast<float>(ZSTEP * zc + c), Br);
                     w[c] = 1.0f;
                 }
             } else {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    const float q = fmaf(A, static_cast<float>(zc + c), B);
+                    const float q = fmaf(A, static_cast<float>(ZSTEP * zc + c), B);
                     const float aq = fabsf(q);
                     const bool vis = aq < 1.0f;
                     const bool band = vis && aq >= 0.6f;
@@ -423,7 +430,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     const size_t nxy = static_cast<size_t>(a.nx) * a.ny;
     float *o = a.out + static_cast<size_t>(xi) * a.ny + yi + static_cast<size_t>(z0) * nxy;
 #pragma unroll
-    for (int z = 0; z < NZ; z++) o[z * nxy] = in_fov ? s[z] * a.out_scale : 0.0f;
+    for (int z = 0; z < NZ; z++) o[(ZSTEP * (z & ~3) + (z & 3)) * nxy] = in_fov ? s[z] * a.out_scale : 0.0f;
 }
 
 template <int NZ, int CPT, int TPR, int ZS>
