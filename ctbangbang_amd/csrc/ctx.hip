@@ -222,7 +222,9 @@ int ensure_maps(ctbb_ctx *ctx, const recon_metadata *mr) {
     return CTBB_OK;
 }
 
-int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out) {
+// n_views_max < 0: every rebinned view of the block; otherwise at most the
+// first n_views_max views.
+int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, int n_views_max = -1) {
     int st = ensure_maps(ctx, mr);
     if (st) return st;
     const ct_geom &cg = mr->cg;
@@ -241,6 +243,7 @@ int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float 
     a.n_proj = static_cast<int>(mr->ri.n_proj_pull / mr->ri.n_ffs);
     a.add = static_cast<int>(cg.add_projections);
     a.n_proj_out = a.n_proj - 2 * a.add;
+    if (n_views_max >= 0 && n_views_max < a.n_proj_out) a.n_proj_out = n_views_max;
     a.reverse_rows = cg.reverse_row_interleave;
     EventPair e = ctx->begin("rebin");
     st = launch_rebin(a, ctx->stream);
@@ -476,9 +479,24 @@ int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out) {
     }
     st = c->rebin_tmp.ensure(sz.rebin * sizeof(float));
     if (st) return st;
-    st = do_rebin(c, mr, win, c->rebin_tmp.as<float>());
+    // The backprojection reads views i + k * half (i < half, k < n_ht) only
+    // (view table, reference src/backproject.cu:39); the pull's remaining
+    // rebinned views (C1: 206 of 1934) are neither rebinned nor filtered.
+    // The FFT filter transforms rows in pairs (2m, 2m + 1) as one complex
+    // signal, so a used row's result depends on its partner: when the used
+    // rows end on an even row, its partner (the next view's first row) is
+    // rebinned and filtered too, and every used value equals the one of the
+    // whole-block filter.
+    const size_t n_out = sz.n_proj_out;
+    size_t views = std::min(n_out, static_cast<size_t>(cg.n_proj_turn / 2) * static_cast<size_t>(bp_n_half_turns(*mr)));
+    size_t rows = views * cg.n_rows;
+    if ((rows & 1) && views < n_out) {
+        views += 1;
+        rows += 1;
+    }
+    st = do_rebin(c, mr, win, c->rebin_tmp.as<float>(), static_cast<int>(views));
     if (st) return st;
-    return do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), sz.n_proj_out * cg.n_rows,
+    return do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), rows,
                      c->rebin_tmp.as<float>(), d_out, c->opt.proj_dtype == CTBB_PROJ_BF16);
 }
 
