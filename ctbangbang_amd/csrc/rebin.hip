@@ -110,6 +110,83 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
     }
 }
 
+// The same arithmetic with 32-bit byte offsets from the (uniform) raw and
+// output bases, one template per channel-tap form (kinds 0/1: bilinear in
+// raw channel and azimuth; 2/3: two oversampled-channel taps, each bilinear
+// in azimuth), for pulls and outputs below 4 GiB.  Few registers (<= 32
+// VGPRs), so its waves fit beside a running backprojection (120 VGPRs,
+// 4 waves/SIMD) and the rebin of the next block overlaps it.
+__device__ __forceinline__ float ld(const float *base, unsigned boff) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff);
+}
+
+template <bool kSplit>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void rebin32_kernel(RebinArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= a.nos) return;
+    const unsigned frame_b = static_cast<unsigned>(a.n_rows_raw * a.nch) * 4u;
+    int set, j;
+    if (a.kind == 0 || a.kind == 2) {
+        set = 0;
+        j = r;
+    } else {
+        set = (r & 1) ^ (a.reverse_rows ? 1 : 0);
+        j = r >> 1;
+    }
+    const float by = a.bidx[set * a.nos + c];
+    int c0, c1;
+    float fc, ao0, ao1 = 0.0f;
+    if (!kSplit) {
+        taps(by, a.nch, c0, c1, fc);
+        ao0 = a.aoff[set * a.nos + c];
+    } else {
+        taps(by, a.nos, c0, c1, fc);
+        ao0 = a.aoff[set * a.nos + c0];
+        ao1 = a.aoff[set * a.nos + c1];
+    }
+    const unsigned row_b = static_cast<unsigned>(j * a.nch) * 4u;
+    const int v0 = blockIdx.z * kViewsPerThread;
+    unsigned ob = (static_cast<unsigned>(v0 * a.n_rows + r) * static_cast<unsigned>(a.nos) + c) * 4u;
+    const unsigned ostep = static_cast<unsigned>(a.n_rows * a.nos) * 4u;
+    for (int vv = 0; vv < kViewsPerThread; vv++) {
+        const int v = v0 + vv;
+        if (v >= a.n_proj_out) break;
+        const float proj = static_cast<float>(v + a.add);
+        float out;
+        if (!kSplit) {
+            const unsigned fs = a.kind == 1 ? static_cast<unsigned>(set) : 0u;
+            int p0, p1;
+            float fa;
+            taps(proj - ao0, a.n_proj, p0, p1, fa);
+            const unsigned b0 = (static_cast<unsigned>(p0 * a.n_ffs) + fs) * frame_b + row_b;
+            const unsigned b1 = (static_cast<unsigned>(p1 * a.n_ffs) + fs) * frame_b + row_b;
+            const float s00 = ld(a.raw, b0 + 4u * c0), s01 = ld(a.raw, b0 + 4u * c1);
+            const float s10 = ld(a.raw, b1 + 4u * c0), s11 = ld(a.raw, b1 + 4u * c1);
+            const float t0 = fmaf(fc, s01 - s00, s00);
+            const float t1 = fmaf(fc, s11 - s10, s10);
+            out = fmaf(fa, t1 - t0, t0);
+        } else {
+            float t[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int m = h ? c1 : c0;
+                const unsigned fs = static_cast<unsigned>(2 * set + (m & 1));
+                const unsigned ch_b = row_b + 4u * static_cast<unsigned>(m >> 1);
+                int p0, p1;
+                float fa;
+                taps(proj - (h ? ao1 : ao0), a.n_proj, p0, p1, fa);
+                const float u0 = ld(a.raw, (static_cast<unsigned>(p0 * a.n_ffs) + fs) * frame_b + ch_b);
+                const float u1 = ld(a.raw, (static_cast<unsigned>(p1 * a.n_ffs) + fs) * frame_b + ch_b);
+                t[h] = fmaf(fa, u1 - u0, u0);
+            }
+            out = fmaf(fc, t[1] - t[0], t[0]);
+        }
+        *reinterpret_cast<float *>(reinterpret_cast<char *>(a.out) + ob) = out;
+        ob += ostep;
+    }
+}
+
 }  // namespace
 
 int launch_rebin(const RebinArgs &a, hipStream_t stream) {
@@ -121,7 +198,16 @@ int launch_rebin(const RebinArgs &a, hipStream_t stream) {
     if (a.n_rows != rows_needed) return CTBB_ERR_SHAPE;
     dim3 grid((a.nos + 255) / 256, a.n_rows, (a.n_proj_out + kViewsPerThread - 1) / kViewsPerThread);
     if (grid.z > 65535u || grid.y > 65535u) return CTBB_ERR_SHAPE;
-    hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, stream, a);
+    const unsigned long long raw_bytes = static_cast<unsigned long long>(a.n_proj) * a.n_ffs * a.n_rows_raw * a.nch * 4ull;
+    const unsigned long long out_bytes = static_cast<unsigned long long>(a.n_proj_out) * a.n_rows * a.nos * 4ull;
+    if (raw_bytes < (1ull << 32) && out_bytes < (1ull << 32)) {
+        if (a.kind <= 1)
+            hipLaunchKernelGGL(rebin32_kernel<false>, grid, dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL(rebin32_kernel<true>, grid, dim3(256), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, stream, a);
+    }
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 
