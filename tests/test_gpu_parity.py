@@ -189,6 +189,37 @@ def test_backproject_random_sinogram(ctx, tmp_path, mini_dir, table_dir, nz):
     cb.free_metadata(mr)
 
 
+@pytest.mark.parametrize("nz", [16, 32])
+@pytest.mark.parametrize("nrows,fov", [(8, 50.0), (16, 50.0), (16, 120.0), (16, 250.0), (32, 60.0), (32, 120.0),
+                                       (64, 40.0), (64, 100.0)])
+def test_backproject_dispatch_shapes(ctx, tmp_path, mini_dir, nrows, fov, nz):
+    """Detector heights 8-64 rows and voxel sizes 1.6-7.8 mm select different
+    staging shapes (threads per row 8-64, channels per thread 1-14, with and
+    without the LDS pad pair, compile-time and generic strides): random
+    sinogram vs the FP64 oracle, all but the rare |q| = 1 steps within 2e-4
+    of the image maximum."""
+    import oracle
+    tok = scans.mini_tokens("none", nrows=nrows, nx=32)
+    tok["ReconFOV"] = fov
+    mr = _mr(tmp_path, tok, mini_dir)
+    cb.plan_block(mr, 0)
+    s = cb.block_sizes(mr)
+    rng = np.random.default_rng(nrows + int(fov))
+    reb = rng.standard_normal((s.n_proj_out, mr.cg.n_rows, mr.cg.n_channels_oversampled)).astype(np.float32)
+    img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
+    print(f"bp shapes: rows {nrows} fov {fov} nz {nz}", flush=True)
+    ctx.set_options(proj_dtype=cb.PROJ_F32, bp_tile=nz)
+    ctx.backproject_dev(mr, dev(reb), img)
+    got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
+    ctx.set_options(bp_tile=0)
+    want = oracle.backproject(mr, reb, oracle.BP_GPU, double=True)
+    err = np.abs(got - want) / np.abs(want).max()
+    n_off = int((err > 2e-4).sum())
+    print(f"bp shapes: rows {nrows} fov {fov} nz {nz}: rel max err {err.max():.3g}, {n_off} voxels beyond 2e-4")
+    assert n_off <= max(2, 1e-4 * err.size)
+    cb.free_metadata(mr)
+
+
 @pytest.mark.parametrize("nz", [32, 16])
 def test_backproject_tall_detector(ctx, tmp_path, mini_dir, nz):
     """128 rebinned rows (z-FFS over 64 detector rows) at a field of view
