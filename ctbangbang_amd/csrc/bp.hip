@@ -433,6 +433,30 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     for (int z = 0; z < NZ; z++) o[(ZSTEP * (z & ~3) + (z & 3)) * nxy] = in_fov ? s[z] * a.out_scale : 0.0f;
 }
 
+// View table of one block, built on the device (the backprojection's work
+// list): entry t = i * n_ht + k (angle i < half, half turn k < n_ht) is
+// theta = tube_start + (2 pi' / n_proj_turn) i + k pi' (reference
+// include/backproject.cuh:75), its sine and cosine, the z term
+// z_rot / 2 - z_rot (theta - tube_start) / (2 pi') and the rebinned
+// projection i + k * half, in FP64 rounded to FP32 once.  Explicitly
+// rounded operations keep the FP64 expression free of contractions.
+__global__ __launch_bounds__(256) void view_table_kernel(BpView *tab, double tube_start, int half, int n_ht,
+                                                          int n_proj_turn, double z_rot) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= half * n_ht) return;
+    const int i = e / n_ht, k = e - i * n_ht;
+    const double pi = static_cast<double>(kPiBp);
+    const double dtheta = __dadd_rn(__dmul_rn(__ddiv_rn(2.0 * pi, static_cast<double>(n_proj_turn)), i),
+                                    __dmul_rn(static_cast<double>(k), pi));
+    const double th = __dadd_rn(tube_start, dtheta);
+    BpView v;
+    v.sin_t = static_cast<float>(sin(th));
+    v.cos_t = static_cast<float>(cos(th));
+    v.c_v = static_cast<float>(__dsub_rn(z_rot / 2.0, __ddiv_rn(__dmul_rn(z_rot, dtheta), 2.0 * pi)));
+    v.proj = i + k * half;
+    tab[e] = v;
+}
+
 template <int NZ, int CPT, int TPR, int ZS>
 int launch_c(const BpArgs &a, hipStream_t stream) {
     const int rows_pad = (256 * ZS) >> a.tpr_log2;
@@ -530,6 +554,16 @@ int launch_backproject(const BpArgs &a, hipStream_t stream) {
     if (a.nz == 32) return launch_nz<32, 1>(a, stream);
     if (a.nz == 16) return launch_nz<16, 2>(a, stream);
     return CTBB_ERR_ARG;
+}
+
+int launch_view_table(BpView *tab, double tube_start, int half, int n_ht, int n_proj_turn, double z_rot,
+                      hipStream_t stream) {
+    if (!tab || half <= 0 || n_ht <= 0 || n_proj_turn <= 0) return CTBB_ERR_ARG;
+    const long n = static_cast<long>(half) * n_ht;
+    if (n > 0x7fffffffL - 255) return CTBB_ERR_SHAPE;
+    hipLaunchKernelGGL(view_table_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, tab,
+                       tube_start, half, n_ht, n_proj_turn, z_rot);
+    return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 
 }  // namespace ctbb

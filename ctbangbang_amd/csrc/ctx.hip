@@ -117,6 +117,39 @@ int upload(ctbb_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
     return hip_ok(hipStreamSynchronize(ctx->stream));
 }
 
+// Backprojection view tables are built on the device, per block, into
+// ctx->vtab: a job holds one block's parameters and its entry offset.
+struct VtabJob {
+    size_t off;
+    double tube_start, z_rot;
+    int half, n_ht, n_proj_turn;
+};
+
+VtabJob vtab_job(const recon_metadata &mr, size_t off) {
+    VtabJob j;
+    j.off = off;
+    j.tube_start = static_cast<double>(bp_tube_start(mr));
+    j.z_rot = static_cast<double>(mr.cg.z_rot);
+    j.half = static_cast<int>(mr.cg.n_proj_turn / 2);
+    j.n_ht = bp_n_half_turns(mr);
+    j.n_proj_turn = static_cast<int>(mr.cg.n_proj_turn);
+    return j;
+}
+
+size_t vtab_len(const VtabJob &j) { return static_cast<size_t>(j.half) * static_cast<size_t>(j.n_ht); }
+
+// Queue the tables of all jobs on the context stream (total entries in all).
+int build_view_tables(ctbb_ctx *ctx, const std::vector<VtabJob> &jobs, size_t total) {
+    int st = ctx->vtab.ensure(total * sizeof(BpView));
+    if (st) return st;
+    for (const VtabJob &j : jobs) {
+        st = launch_view_table(ctx->vtab.as<BpView>() + j.off, j.tube_start, j.half, j.n_ht, j.n_proj_turn, j.z_rot,
+                               ctx->stream);
+        if (st) return st;
+    }
+    return CTBB_OK;
+}
+
 // Zero fill on a stream (a kernel rather than hipMemsetAsync, so the
 // sequence stays valid inside a captured graph).
 __global__ void fill_zero_kernel(float *p, size_t n) {
@@ -454,9 +487,9 @@ int backproject_dev(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin
     int st = check_geometry(mr);
     if (st) return st;
     if (!mr->tube_angles) return CTBB_ERR_ARG;
-    std::vector<BpView> tab;
-    bp_view_table(*mr, tab);
-    st = upload(ctx, ctx->vtab, tab.data(), tab.size() * sizeof(BpView));
+    const VtabJob j = vtab_job(*mr, 0);
+    if (j.n_ht <= 0) return CTBB_ERR_SHAPE;
+    st = build_view_tables(ctx, {j}, vtab_len(j));
     if (st) return st;
     return do_backproject(ctx, mr, d_rebin, ctx->vtab.as<BpView>(), d_img);
 }
@@ -733,19 +766,20 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
         if (st) return st;
         d_image = c->image_stack.as<float>();
     }
-    // Per-block view tables, computed up front and uploaded once.
-    std::vector<BpView> all;
+    // Per-block view tables, built on the device up front.
+    std::vector<VtabJob> jobs;
     std::vector<size_t> off(n_blocks);
+    size_t total = 0;
     for (int b = 0; b < n_blocks; b++) {
         ctbb::plan_block(mr, b);
         st = check_geometry(mr);
         if (st) return st;
-        std::vector<BpView> t;
-        bp_view_table(*mr, t);
-        off[b] = all.size();
-        all.insert(all.end(), t.begin(), t.end());
+        jobs.push_back(vtab_job(*mr, total));
+        if (jobs.back().n_ht <= 0) return CTBB_ERR_SHAPE;
+        off[b] = total;
+        total += vtab_len(jobs.back());
     }
-    st = upload(c, c->vtab, all.data(), all.size() * sizeof(BpView));
+    st = build_view_tables(c, jobs, total);
     if (st) return st;
     st = ensure_taps(c, mr);
     if (st) return st;
@@ -806,23 +840,24 @@ int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h
     int st = c->image_stack.ensure(mr->ri.n_slices_recon * nxy * sizeof(float));
     if (st) return st;
     float *d_image = c->image_stack.as<float>();
-    std::vector<BpView> all;
+    std::vector<VtabJob> jobs;
     std::vector<size_t> off(n_blocks);
+    size_t total = 0;
     std::vector<long> start(n_blocks), count(n_blocks);
     size_t max_pull = 0;
     for (int b = 0; b < n_blocks; b++) {
         ctbb::plan_block(mr, b);
         st = check_geometry(mr);
         if (st) return st;
-        std::vector<BpView> t;
-        bp_view_table(*mr, t);
-        off[b] = all.size();
-        all.insert(all.end(), t.begin(), t.end());
+        jobs.push_back(vtab_job(*mr, total));
+        if (jobs.back().n_ht <= 0) return CTBB_ERR_SHAPE;
+        off[b] = total;
+        total += vtab_len(jobs.back());
         start[b] = static_cast<long>(static_cast<int>(mr->ri.idx_pull_start));
         count[b] = static_cast<long>(mr->ri.n_proj_pull);
         max_pull = std::max(max_pull, static_cast<size_t>(count[b]));
     }
-    st = upload(c, c->vtab, all.data(), all.size() * sizeof(BpView));
+    st = build_view_tables(c, jobs, total);
     if (st) return st;
     st = ensure_taps(c, mr);
     if (st) return st;
@@ -950,9 +985,10 @@ int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *
         }
     }
     c->reset_events();
-    // view tables of every (scan, block) and finalize tap lists of every scan,
-    // uploaded once
-    std::vector<BpView> all;
+    // view tables of every (scan, block), built on the device, and finalize
+    // tap lists of every scan, uploaded once
+    std::vector<VtabJob> jobs;
+    size_t total = 0;
     std::vector<std::vector<size_t>> off(n_scans);
     std::vector<char> fin;
     std::vector<size_t> fin_off(n_scans);
@@ -966,10 +1002,10 @@ int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *
             ctbb::plan_block(m, b);
             st = check_geometry(m);
             if (st) return st;
-            std::vector<BpView> t;
-            bp_view_table(*m, t);
-            off[s][b] = all.size();
-            all.insert(all.end(), t.begin(), t.end());
+            jobs.push_back(vtab_job(*m, total));
+            if (jobs.back().n_ht <= 0) return CTBB_ERR_SHAPE;
+            off[s][b] = total;
+            total += vtab_len(jobs.back());
             ctbb_sizes sz;
             ctbb_block_sizes(m, &sz);
             max_rebin = std::max(max_rebin, sz.rebin);
@@ -982,7 +1018,7 @@ int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *
         fin.resize(fin_off[s] + blob.size());
         std::memcpy(fin.data() + fin_off[s], blob.data(), blob.size());
     }
-    st = upload(c, c->vtab, all.data(), all.size() * sizeof(BpView));
+    st = build_view_tables(c, jobs, total);
     if (st) return st;
     st = upload(c, c->fin_tab, fin.data(), fin.size());
     if (st) return st;

@@ -67,29 +67,6 @@ int bp_n_half_turns(const recon_metadata &mr) {
     return static_cast<int>((mr.ri.n_proj_pull / mr.ri.n_ffs - 2 * cg.add_projections) / (cg.n_proj_turn / 2));
 }
 
-void bp_view_table(const recon_metadata &mr, std::vector<BpView> &tab) {
-    const ct_geom &cg = mr.cg;
-    const int half = static_cast<int>(cg.n_proj_turn / 2);
-    const int n_ht = bp_n_half_turns(mr);
-    const double pi = static_cast<double>(kPiBp);
-    const double ts = static_cast<double>(bp_tube_start(mr));
-    const double zr = cg.z_rot;
-    tab.resize(static_cast<size_t>(half) * n_ht);
-    for (int i = 0; i < half; i++)
-        for (int k = 0; k < n_ht; k++) {
-            // theta = tube_start + (2 pi'/n_proj_turn) i + k pi'  (backproject.cuh:75)
-            double dtheta = (2.0 * pi / static_cast<double>(cg.n_proj_turn)) * i + k * pi;
-            double th = ts + dtheta;
-            BpView v;
-            v.sin_t = static_cast<float>(std::sin(th));
-            v.cos_t = static_cast<float>(std::cos(th));
-            // z - ray_pos = zi*coll + z_rot/2 + z_rot*(tube_start - theta)/(2 pi') + z_rot*asin(phat/r_f)/(2 pi')
-            v.c_v = static_cast<float>(zr / 2.0 - zr * dtheta / (2.0 * pi));
-            v.proj = i + k * half;
-            tab[static_cast<size_t>(i) * n_ht + k] = v;
-        }
-}
-
 int rebin_maps(const recon_metadata &mr, std::vector<float> &bidx, std::vector<float> &aoff, int *kind,
                int *n_sets) {
     const ct_geom &cg = mr.cg;

@@ -10,8 +10,6 @@ namespace ctbb {
 void ffs_offsets(const recon_metadata &mr, double *da, double *dr);
 float bp_tube_start(const recon_metadata &mr);
 int bp_n_half_turns(const recon_metadata &mr);
-// tab[k*half + i] for view v = i + k*half (half = n_proj_turn/2)
-void bp_view_table(const recon_metadata &mr, std::vector<BpView> &tab);
 // kind: 0 nffs, 1 zffs, 2 pffs, 3 affs; maps laid out [set][n_channels_oversampled]
 int rebin_maps(const recon_metadata &mr, std::vector<float> &bidx, std::vector<float> &aoff, int *kind,
                int *n_sets);

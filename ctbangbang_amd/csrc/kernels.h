@@ -85,6 +85,10 @@ struct FinalizeArgs {
 };
 int launch_finalize(const FinalizeArgs &a, hipStream_t stream);
 
+// Backprojection view table of one block (half * n_ht entries) on the device
+int launch_view_table(BpView *tab, double tube_start, int half, int n_ht, int n_proj_turn, double z_rot,
+                      hipStream_t stream);
+
 // HU -> int16 stored pixel values (HU = slope * stored + intercept)
 struct Hu16Args {
     const float *in;

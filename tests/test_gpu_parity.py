@@ -67,7 +67,8 @@ def test_ramp_filter(ctx, mode, tol, n):
     want = oracle.filter_fft(x, taps)
     ctx.set_options(filter_mode=mode, proj_dtype=cb.PROJ_F32)
     out = torch.empty(rows * n, dtype=torch.float32, device="cuda")
-    ctx.filter_rows_dev(dev(taps), n, rows, dev(x), out)
+    taps_d, x_d = dev(taps), dev(x)
+    ctx.filter_rows_dev(taps_d, n, rows, x_d, out)
     got = host(out).reshape(rows, n)
     err = np.abs(got - want).max() / np.abs(want).max()
     print(f"filter n={n} mode={mode}: rel max err {err:.3g}")
@@ -86,8 +87,9 @@ def test_ramp_filter_fft_in_place(ctx):
     ctx.set_options(filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
     a = dev(x)
     out = torch.empty(rows * n, dtype=torch.float32, device="cuda")
-    ctx.filter_rows_dev(dev(taps), n, rows, a, out)
-    ctx.filter_rows_dev(dev(taps), n, rows, a, a)
+    taps_d = dev(taps)
+    ctx.filter_rows_dev(taps_d, n, rows, a, out)
+    ctx.filter_rows_dev(taps_d, n, rows, a, a)
     ctx.set_options(filter_mode=cb.FILTER_F32)
     assert np.array_equal(host(a).ravel(), host(out))
 
@@ -105,7 +107,8 @@ def test_ramp_filter_bf16_output(ctx, mode):
     want = oracle.filter_fft(x, taps)
     ctx.set_options(filter_mode=mode, proj_dtype=cb.PROJ_BF16)
     out = torch.empty(rows * n, dtype=torch.int16, device="cuda")
-    ctx.filter_rows_dev(dev(taps), n, rows, dev(x), out)
+    taps_d, x_d = dev(taps), dev(x)
+    ctx.filter_rows_dev(taps_d, n, rows, x_d, out)
     ctx.set_options(filter_mode=cb.FILTER_F32, proj_dtype=cb.PROJ_F32)
     bits = host(out).astype(np.uint16).astype(np.uint32) << 16
     got = bits.view(np.float32).reshape(rows, n)
@@ -122,7 +125,8 @@ def test_filter_impulse_gives_taps(ctx):
         x[r, (r * 23) % n] = 1.0
     out = torch.empty(64 * n, dtype=torch.float32, device="cuda")
     ctx.set_options(filter_mode=cb.FILTER_F32)
-    ctx.filter_rows_dev(dev(taps), n, 64, dev(x), out)
+    taps_d, x_d = dev(taps), dev(x)
+    ctx.filter_rows_dev(taps_d, n, 64, x_d, out)
     got = host(out).reshape(64, n)
     for r in range(64):
         k = (r * 23) % n
@@ -179,7 +183,8 @@ def test_backproject_random_sinogram(ctx, tmp_path, mini_dir, table_dir, nz):
     reb = rng.standard_normal((s.n_proj_out, mr.cg.n_rows, mr.cg.n_channels_oversampled)).astype(np.float32)
     img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
     ctx.set_options(proj_dtype=cb.PROJ_F32, bp_tile=nz)
-    ctx.backproject_dev(mr, dev(reb), img)
+    reb_d = dev(reb)
+    ctx.backproject_dev(mr, reb_d, img)
     got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
     ctx.set_options(bp_tile=0)
     want = oracle.backproject(mr, reb, oracle.BP_GPU, double=True)
@@ -209,7 +214,8 @@ def test_backproject_dispatch_shapes(ctx, tmp_path, mini_dir, nrows, fov, nz):
     img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
     print(f"bp shapes: rows {nrows} fov {fov} nz {nz}", flush=True)
     ctx.set_options(proj_dtype=cb.PROJ_F32, bp_tile=nz)
-    ctx.backproject_dev(mr, dev(reb), img)
+    reb_d = dev(reb)
+    ctx.backproject_dev(mr, reb_d, img)
     got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
     ctx.set_options(bp_tile=0)
     want = oracle.backproject(mr, reb, oracle.BP_GPU, double=True)
@@ -243,7 +249,8 @@ def test_backproject_tall_detector(ctx, tmp_path, mini_dir, nz):
     reb = rng.standard_normal((s.n_proj_out, mr.cg.n_rows, mr.cg.n_channels_oversampled)).astype(np.float32)
     img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
     ctx.set_options(proj_dtype=cb.PROJ_F32, bp_tile=nz)
-    ctx.backproject_dev(mr, dev(reb), img)
+    reb_d = dev(reb)
+    ctx.backproject_dev(mr, reb_d, img)
     got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
     ctx.set_options(bp_tile=0)
     want = oracle.backproject(mr, reb, oracle.BP_GPU, double=True)
@@ -310,7 +317,8 @@ def test_backproject_constant_sinogram(ctx, tmp_path, mini_dir):
     s = cb.block_sizes(mr)
     reb = np.full((s.n_proj_out, mr.cg.n_rows, mr.cg.n_channels_oversampled), 2.5, dtype=np.float32)
     img = torch.empty(s.image_block, dtype=torch.float32, device="cuda")
-    ctx.backproject_dev(mr, dev(reb), img)
+    reb_d = dev(reb)
+    ctx.backproject_dev(mr, reb_d, img)
     got = host(img).reshape(32, mr.rp.nx, mr.rp.ny)
     want = np.float32(3.1415368979) * 2.5
     assert np.abs(got - want).max() <= 2e-5 * want
@@ -329,7 +337,8 @@ def test_finalize_bit_exact(ctx, tmp_path, mini_dir):
     out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
     for mu in (0.0, 0.02):
         ctx.set_options(mu_water=mu)
-        ctx.finalize_dev(mr, dev(img), out)
+        img_d = dev(img)
+        ctx.finalize_dev(mr, img_d, out)
         got = host(out).reshape(-1, mr.rp.nx, mr.rp.ny)
         want = oracle.finalize(mr, img, mu, double=False)
         assert np.array_equal(got, want), mu
@@ -409,10 +418,14 @@ def test_pipeline_matches_stage_chain(ctx, tmp_path, mini_dir, ffs):
         win = raw[s0 * frame:(s0 + n) * frame]
         af = torch.empty_like(win)
         ctx.adaptive_filter_dev(mr, win, af)
+        # the context's streams are not torch's: wait before torch reads af
+        # or frees a buffer the queued work still uses
+        ctx.synchronize()
         changed += int((af != win).sum())
         reb = torch.empty(sb.rebin, dtype=torch.float32, device="cuda")
         ctx.rebin_filter_dev(mr, af, reb)
         ctx.backproject_dev(mr, reb, stack[b * sb.image_block:(b + 1) * sb.image_block])
+        ctx.synchronize()
     want = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
     ctx.finalize_dev(mr, stack, want)
     want = host(want)
@@ -477,6 +490,7 @@ def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir):
             ctx.reconstruct_dev(mr, raw, n, out)
             singles.append(host(out).copy())
             out.zero_()
+        torch.cuda.synchronize()    # torch's zero fill before the context's streams write out
         ctx.reconstruct_batch_dev([i[0] for i in items], [i[1] for i in items], [i[2] for i in items],
                                   [i[3] for i in items])
         ctx.synchronize()
