@@ -9,8 +9,10 @@
 // The reference's texture fetches are replaced by explicit FP32 bilinear
 // interpolation from an LDS-staged projection footprint.
 //
-// Mapping: one workgroup = 16x16 voxel columns x NZ slices; one thread = one
-// column (xi, yi) with its NZ slice accumulators in registers.  Views are
+// Mapping: one workgroup = 16x16 voxel columns x the block's 32 slices, as
+// two z-groups of NZ = 16 slices (interleaved by chunks of 4) or one of 32;
+// one thread = one column (xi, yi) with its NZ slice accumulators in
+// registers.  Views are
 // processed in the reference order (angle i outer, half turn k inner) through
 // a two-stage pipeline: while the workgroup interpolates view t from one LDS
 // buffer, the rows x W channel window of view t+1 is already in flight from
