@@ -94,6 +94,7 @@ _SIGS = {
     "ctbb_filter_rows_dev": (C.c_int, [_VP, _VP, C.c_int, C.c_size_t, _VP, _VP]),
     "ctbb_backproject_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_finalize_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
+    "ctbb_autotune_backprojection": (C.c_int, [_VP, _MR, C.c_char_p, _P(C.c_int)]),
     "ctbb_hu_to_int16_dev": (C.c_int, [_VP, _VP, C.c_size_t, C.c_float, C.c_float, _VP]),
     "ctbb_write_dicom_series": (C.c_int, [_MR, _VP, C.c_float, C.c_float, C.c_char_p, C.c_char_p]),
     "ctbb_reconstruct_dev": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP, _VP]),
@@ -307,6 +308,15 @@ class Context:
 
     def finalize_dev(self, mr, d_image, d_final):
         _check(lib().ctbb_finalize_dev(self._h, C.byref(mr), _ptr(d_image), _ptr(d_final)), "finalize_dev")
+
+    def autotune_backprojection(self, mr, cache_path=None):
+        """Choose bp_tile (16 or 32) by timing the middle block; sets it in the
+        context's options and returns it.  cache_path: text cache per shape."""
+        t = C.c_int()
+        _check(lib().ctbb_autotune_backprojection(self._h, C.byref(mr),
+                                                  None if cache_path is None else os.fsencode(cache_path),
+                                                  C.byref(t)), "autotune_backprojection")
+        return t.value
 
     def hu_to_int16_dev(self, d_hu, n, d_out, slope=1.0, intercept=0.0):
         """int16 stored values of a HU stack (HU = slope * stored + intercept):

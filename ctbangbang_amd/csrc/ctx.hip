@@ -732,6 +732,76 @@ int ctbb_backproject_dev(ctbb_ctx *c, const struct recon_metadata *mr, const voi
     return backproject_dev(c, mr, d_rebin, d_image_block);
 }
 
+// Backprojection autotuner: bp_tile 16 (two z-groups) or 32 (one group).
+// The shape that decides it is the staged window (rows, oversampled channels,
+// voxel size through the footprint), the matrix, the number of views and the
+// projection type; results are cached per shape in a text file of lines
+// "<key> <tile>".
+int ctbb_autotune_backprojection(ctbb_ctx *c, struct recon_metadata *mr, const char *cache_path, int *bp_tile_out) {
+    if (!c || !mr || !mr->tube_angles || !bp_tile_out) return CTBB_ERR_ARG;
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
+    if (mr->ri.n_blocks <= 0) return CTBB_ERR_ARG;
+    ctbb::plan_block(mr, static_cast<int>(mr->ri.n_blocks / 2));
+    int st = check_geometry(mr);
+    if (st) return st;
+    char key[256];
+    std::snprintf(key, sizeof(key), "rows%zu_ch%zu_nx%u_ny%u_fov%.4f_ht%d_half%d_%s", mr->cg.n_rows,
+                  mr->cg.n_channels_oversampled, mr->rp.nx, mr->rp.ny, static_cast<double>(mr->rp.recon_fov),
+                  bp_n_half_turns(*mr), static_cast<int>(mr->cg.n_proj_turn / 2),
+                  c->opt.proj_dtype == CTBB_PROJ_BF16 ? "bf16" : "f32");
+    int best = 0;
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "r")) {
+            char k[256];
+            int t = 0;
+            while (std::fscanf(f, "%255s %d", k, &t) == 2)
+                if (std::strcmp(k, key) == 0 && (t == 16 || t == 32)) best = t;
+            std::fclose(f);
+        }
+    }
+    if (!best) {
+        ctbb_sizes sz;
+        ctbb_block_sizes(mr, &sz);
+        const size_t reb_floats = (sz.rebin * proj_elem_bytes(c) + 3) / 4;
+        DevBuf reb, img;
+        const int keep = c->opt.bp_tile;
+        float best_ms = 0.0f;
+        // the work does not depend on the values: a zero block
+        if (!(st = reb.ensure(reb_floats * sizeof(float))) && !(st = img.ensure(sz.image_block * sizeof(float))))
+            st = fill_zero(reb.as<float>(), reb_floats, c->stream);
+        for (int tile : {16, 32}) {
+            if (st) break;
+            c->opt.bp_tile = tile;
+            st = backproject_dev(c, mr, reb.p, img.as<float>());          // warm-up
+            if (!st) st = hip_ok(hipStreamSynchronize(c->stream));
+            c->reset_events();
+            for (int r = 0; r < 3 && !st; r++) st = backproject_dev(c, mr, reb.p, img.as<float>());
+            float ms = 0.0f;
+            int n = 0;
+            if (!st) st = ctbb_ctx_kernel_time(c, "backproject", &ms, &n);
+            if (!st && n > 0 && (!best || ms / n < best_ms)) {
+                best = tile;
+                best_ms = ms / n;
+            }
+        }
+        c->opt.bp_tile = keep;
+        if (!st) st = hip_ok(hipStreamSynchronize(c->stream));
+        reb.release();
+        img.release();
+        if (st) return st;
+        if (!best) return CTBB_ERR_HIP;
+        if (cache_path) {
+            FILE *f = std::fopen(cache_path, "a");
+            if (!f) return CTBB_ERR_IO;
+            std::fprintf(f, "%s %d\n", key, best);
+            if (std::fclose(f) != 0) return CTBB_ERR_IO;
+        }
+    }
+    c->opt.bp_tile = best;
+    *bp_tile_out = best;
+    return CTBB_OK;
+}
+
 int ctbb_finalize_dev(ctbb_ctx *c, const struct recon_metadata *mr, const float *d_image, float *d_final) {
     if (!c || !mr || !d_image || !d_final) return CTBB_ERR_ARG;
     if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;

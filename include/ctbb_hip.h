@@ -225,6 +225,20 @@ int ctbb_backproject_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
 int ctbb_finalize_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
                       const float *d_image, float *d_final);
 
+/* Backprojection autotuner: times one launch per candidate bp_tile (16: two
+ * z-groups of 16 slices per workgroup; 32: one group of 32) for the middle
+ * block of the plan in mr (on a zero-filled block: the work does not depend
+ * on the values), sets the faster in the context's options and returns it in
+ * *bp_tile_out.  With cache_path, results are kept per shape (rows, channels,
+ * matrix, field of view, views, projection type) as lines "<key> <tile>" in
+ * that text file, and a cached shape is used without timing.  Both tiles
+ * give the same images to FP32 rounding (the interior-chunk shortcut is
+ * decided per wavefront, whose voxels and slices differ between the two).
+ * No counterpart in the reference, whose tile is a compile-time constant
+ * (include/backproject.cuh:24-25). */
+int ctbb_autotune_backprojection(ctbb_ctx *ctx, struct recon_metadata *mr, const char *cache_path,
+                                 int *bp_tile_out);
+
 /* Output format: int16 stored values of a stack in HU, for a DICOM series
  * (HU = slope * stored + intercept, the Rescale Slope / Intercept of the
  * files): d_out[i] = (d_hu[i] - intercept) / slope in IEEE FP32, rounded half

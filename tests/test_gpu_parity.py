@@ -665,6 +665,40 @@ def test_dicom_series_from_reconstruction(ctx, tmp_path, mini_dir):
     cb.free_metadata(mr)
 
 
+def test_autotune_backprojection(ctx, tmp_path, mini_dir):
+    """The tile autotuner picks 16 or 32, sets it, records it per shape in
+    its cache and reuses the cached choice; reconstructions with either tile
+    agree to FP32 rounding."""
+    tok = scans.mini_tokens("z", nrows=8, nx=32)
+    mr = _mr(tmp_path, tok, mini_dir)
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32,
+                    bp_tile=0)
+    cache = str(tmp_path / "tune.txt")
+    t1 = ctx.autotune_backprojection(mr, cache)
+    assert t1 in (16, 32) and ctx.options().bp_tile == t1
+    text = open(cache).read()
+    assert text.count("\n") == 1 and text.split()[1] == str(t1)
+    ctx.set_options(bp_tile=0)
+    assert ctx.autotune_backprojection(mr, cache) == t1
+    assert open(cache).read() == text                   # cached: not timed again
+    n_read = tok["Readings"]
+    frame = mr.cg.n_rows_raw * mr.cg.n_channels
+    raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda")
+    ctx.synth_scan_dev(mr, scans.phantom_for(tok, 3), 0, n_read, raw)
+    s = cb.block_sizes(mr)
+    outs = []
+    for tile in (16, 32):
+        ctx.set_options(bp_tile=tile)
+        out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+        ctx.reconstruct_dev(mr, raw, n_read, out)
+        outs.append(host(out))
+    ctx.set_options(bp_tile=0, mu_water=0.0)
+    d = np.abs(outs[0].astype(np.float64) - outs[1])
+    print(f"autotune: tile {t1}; tile 16 vs 32 max HU difference {d.max():.2e}")
+    assert d.max() <= 1e-3       # HU
+    cb.free_metadata(mr)
+
+
 def test_context_options_validated(ctx):
     """Unsupported option values are rejected and leave the options unchanged."""
     before = ctx.options()
