@@ -155,20 +155,11 @@ __global__ __launch_bounds__(64) void af_thresh(const float *sm, const float *mx
 }
 
 // reference preprocessing.cuh:208-285 (3D branch, __FILTER_3D__ defined)
-__global__ __launch_bounds__(256) void af_apply(const float *raw, const float *thr, float *out, int n_proj,
-                                                int n_rows_raw, int nch) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // channel
-    const int j = blockIdx.y;                                // row
-    const int p = blockIdx.z;                                // view
-    if (i >= nch) return;
-    const size_t ps = static_cast<size_t>(nch) * n_rows_raw;
-    const size_t idx = ps * p + static_cast<size_t>(j) * nch + i;
-    const float T = thr[p];
-    const float x = raw[idx];
-    if (!(x > T)) {
-        out[idx] = x;
-        return;
-    }
+// Output sample (i, j, p): copied when x <= T, else the 5x5x5 weighted
+// smoothing of its neighbourhood.
+__device__ __forceinline__ float af_value(const float *__restrict__ raw, float x, float T, int i, int j, int p,
+                                          int n_proj, int n_rows_raw, int nch, size_t ps) {
+    if (!(x > T)) return x;
     float norm = 0.0f, val = 0.0f;
     for (int ii = 0; ii < 5; ii++) {
         const int fc = i + ii - 2;
@@ -184,7 +175,44 @@ __global__ __launch_bounds__(256) void af_apply(const float *raw, const float *t
             }
         }
     }
-    out[idx] = (1.0f / norm) * val;
+    return (1.0f / norm) * val;
+}
+
+// One thread per sample (low register use: runs beside a backprojection).
+__global__ __launch_bounds__(256) void af_apply(const float *raw, const float *thr, float *out, int n_proj,
+                                                int n_rows_raw, int nch) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // channel
+    const int j = blockIdx.y;                                // row
+    const int p = blockIdx.z;                                // view
+    if (i >= nch) return;
+    const size_t ps = static_cast<size_t>(nch) * n_rows_raw;
+    const size_t idx = ps * p + static_cast<size_t>(j) * nch + i;
+    out[idx] = af_value(raw, raw[idx], thr[p], i, j, p, n_proj, n_rows_raw, nch, ps);
+}
+
+// Four consecutive samples of a view's frame per thread, 16-byte loads and
+// stores (for a GPU that runs nothing else: the first block of a pipeline
+// and the stage entry point).
+__global__ __launch_bounds__(256) void af_apply4(const float *raw, const float *thr, float *out, int n_proj,
+                                                 int n_rows_raw, int nch) {
+    const int p = blockIdx.y;
+    const size_t ps = static_cast<size_t>(nch) * n_rows_raw;
+    const int e = (blockIdx.x * blockDim.x + threadIdx.x) * 4;   // sample in the frame
+    if (static_cast<size_t>(e) >= ps) return;
+    const size_t base = ps * p + e;
+    const float T = thr[p];
+    const float4 x = *reinterpret_cast<const float4 *>(raw + base);
+    float4 y = x;
+    if (x.x > T || x.y > T || x.z > T || x.w > T) {
+        float *yv = reinterpret_cast<float *>(&y);
+        const float xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = (e + q) / nch, i = e + q - j * nch;
+            yv[q] = af_value(raw, xv[q], T, i, j, p, n_proj, n_rows_raw, nch, ps);
+        }
+    }
+    *reinterpret_cast<float4 *>(out + base) = y;
 }
 
 int odd_window(float frac, int n_proj_ffs) {
@@ -216,8 +244,13 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     else
         hipLaunchKernelGGL(af_thresh<false>, dim3(a.n_proj), dim3(64), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
                            n_range, a.n_proj_ffs, a.strength);
-    hipLaunchKernelGGL(af_apply, dim3((a.nch + 255) / 256, a.n_rows_raw, a.n_proj), dim3(256), 0, stream, a.raw,
-                       thr, a.out, a.n_proj, a.n_rows_raw, a.nch);
+    const bool aligned = (reinterpret_cast<uintptr_t>(a.raw) & 15u) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 15u) == 0;
+    if (a.vec4 && aligned)
+        hipLaunchKernelGGL(af_apply4, dim3(static_cast<unsigned>((frame / 4 + 255) / 256), a.n_proj), dim3(256), 0,
+                           stream, a.raw, thr, a.out, a.n_proj, a.n_rows_raw, a.nch);
+    else
+        hipLaunchKernelGGL(af_apply, dim3((a.nch + 255) / 256, a.n_rows_raw, a.n_proj), dim3(256), 0, stream, a.raw,
+                           thr, a.out, a.n_proj, a.n_rows_raw, a.nch);
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 

@@ -446,7 +446,9 @@ int do_finalize(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_image, f
     return launch_finalize_blob(ctx, mr, d_image, d_final, ctx->fin_tab.as<char>(), mt, nf, ctx->stream);
 }
 
-int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out) {
+// vec4: the 16-byte apply kernel (for a GPU that runs nothing else; the
+// one-sample kernel needs few registers and runs beside a backprojection).
+int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, bool vec4 = true) {
     const ct_geom &cg = mr->cg;
     AdaptiveArgs a{};
     a.raw = d_raw;
@@ -456,6 +458,7 @@ int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, flo
     a.nch = static_cast<int>(cg.n_channels);
     a.n_proj_ffs = static_cast<int>(cg.n_proj_ffs);
     a.strength = mr->rp.adaptive_filtration_s;
+    a.vec4 = vec4 ? 1 : 0;
     int st = ctx->af_work.ensure(6 * static_cast<size_t>(a.n_proj) * sizeof(float));
     if (st) return st;
     a.work = ctx->af_work.as<float>();
@@ -496,7 +499,8 @@ int backproject_dev(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin
 
 // Adaptive filter -> rebin -> ramp filter of the current block (mr->ri.cb)
 // on ctx->stream, raw pull at `win`, filtered projections into d_out.
-int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out) {
+// first: the first item of a pipeline (no backprojection is running yet).
+int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out, bool first) {
     const ct_geom &cg = mr->cg;
     const size_t frame = cg.n_rows_raw * cg.n_channels;
     const size_t n = mr->ri.n_proj_pull;
@@ -506,7 +510,7 @@ int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out) {
     if (c->opt.adaptive_filter) {
         st = c->raw_af.ensure(n * frame * sizeof(float));
         if (st) return st;
-        st = do_adaptive(c, mr, win, c->raw_af.as<float>());
+        st = do_adaptive(c, mr, win, c->raw_af.as<float>(), first);
         if (st) return st;
         win = c->raw_af.as<float>();
     }
@@ -563,7 +567,7 @@ int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int item, int b, const float
     DevBuf &out = c->filt[k];
     if (item >= ctbb_ctx::kPipeDepth && hipStreamWaitEvent(c->stream, c->ev_bp[k], 0) != hipSuccess)
         return CTBB_ERR_HIP;
-    int st = prep_block(c, mr, win, out.p);
+    int st = prep_block(c, mr, win, out.p, item == 0);
     if (st) return st;
     if (hipEventRecord(c->ev_filt[k], c->stream) != hipSuccess ||
         hipStreamWaitEvent(c->stream_bp, c->ev_filt[k], 0) != hipSuccess)

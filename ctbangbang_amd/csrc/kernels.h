@@ -69,6 +69,7 @@ struct AdaptiveArgs {
     int n_proj, n_rows_raw, nch;
     int n_proj_ffs;
     float strength;
+    int vec4;                  // 1: 16-byte apply kernel where aligned (GPU otherwise idle)
 };
 int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream);
 
