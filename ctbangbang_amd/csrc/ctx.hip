@@ -745,6 +745,14 @@ int ctbb_autotune_backprojection(ctbb_ctx *c, struct recon_metadata *mr, const c
     if (!c || !mr || !mr->tube_angles || !bp_tile_out) return CTBB_ERR_ARG;
     if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     if (mr->ri.n_blocks <= 0) return CTBB_ERR_ARG;
+    // plan the middle block for the measurement; the caller's block state
+    // (mr->ri) is restored on return
+    const recon_info saved_ri = mr->ri;
+    struct Restore {
+        recon_metadata *m;
+        const recon_info &ri;
+        ~Restore() { m->ri = ri; }
+    } restore{mr, saved_ri};
     ctbb::plan_block(mr, static_cast<int>(mr->ri.n_blocks / 2));
     int st = check_geometry(mr);
     if (st) return st;

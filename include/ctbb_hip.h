@@ -231,7 +231,8 @@ int ctbb_finalize_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
  * on the values), sets the faster in the context's options and returns it in
  * *bp_tile_out.  With cache_path, results are kept per shape (rows, channels,
  * matrix, field of view, views, projection type) as lines "<key> <tile>" in
- * that text file, and a cached shape is used without timing.  Both tiles
+ * that text file, and a cached shape is used without timing.  mr->ri is
+ * left as it was.  Both tiles
  * give the same images to FP32 rounding (the interior-chunk shortcut is
  * decided per wavefront, whose voxels and slices differ between the two).
  * No counterpart in the reference, whose tile is a compile-time constant

@@ -674,8 +674,11 @@ def test_autotune_backprojection(ctx, tmp_path, mini_dir):
     ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32,
                     bp_tile=0)
     cache = str(tmp_path / "tune.txt")
+    cb.plan_block(mr, 0)
+    before = (int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull), int(mr.ri.cb.block_idx))
     t1 = ctx.autotune_backprojection(mr, cache)
     assert t1 in (16, 32) and ctx.options().bp_tile == t1
+    assert (int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull), int(mr.ri.cb.block_idx)) == before
     text = open(cache).read()
     assert text.count("\n") == 1 and text.split()[1] == str(t1)
     ctx.set_options(bp_tile=0)
