@@ -332,7 +332,9 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         if (t + 1 < n_views) {
             const float4 vn = ring[(t + 1) & 127];
             plo_next = window_lo(vn);
+            __builtin_amdgcn_s_setprio(1);
             fetch(__float_as_int(vn.w), plo_next);
+            __builtin_amdgcn_s_setprio(0);
             if (ZS == 2 && zg == ((t + 1) & 1)) {
                 float fpn, gn, dzn;
                 unsigned cin;
@@ -396,6 +398,10 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
                 }
             }
             float2 e0[4], e1[4];
+            // the chunk's LDS reads (and, below, the next view's window
+            // loads) are issued at raised wave priority, so waves start their
+            // long-latency requests ahead of other waves' arithmetic
+            __builtin_amdgcn_s_setprio(1);
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const unsigned idx = colb + __umul24(static_cast<unsigned>(rowf[c]), static_cast<unsigned>(RS));
@@ -404,6 +410,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
             }
             // all 8 reads of the chunk are issued before the first is used
             __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_setprio(0);
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const float fr = __builtin_amdgcn_fractf(rowf[c]);
