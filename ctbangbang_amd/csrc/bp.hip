@@ -307,6 +307,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         if (++k == a.n_ht) k = 0;
         const int p_lo = plo_next;
         float2 *buf = lds2 + (t & 1) * n_stage;
+        __builtin_amdgcn_s_setprio(1);     // staging writes, ahead of other waves' arithmetic
         {
             float2 *dst = buf + tid * (CPT + PAD);      // = srow * RS + scol + PAD * (scol / CPT)
             float prev = bits_to_f32<kBf16>(pv[0]);
@@ -317,6 +318,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
                 prev = nxt;
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if ((t & 63) == 0 && t + 64 < n_views) load_chunk((t >> 6) + 1);
         float fp, g, dz;
