@@ -61,6 +61,11 @@ static const float PI_TRUE = 3.1415926535897f;
 
 #define ORC_BP_GPU_CONVENTIONS 0
 #define ORC_BP_CPU_CONVENTIONS 1
+/* Planted mistakes for the ground-truth tests (bits added to the GPU
+ * conventions): drop the x flip, drop the -table_direction sign of qhat.
+ * A test that plants one must see its checks fail. */
+#define ORC_BP_PLANT_NO_FLIP 2
+#define ORC_BP_PLANT_NO_QSIGN 4
 
 /* ------------------------------------------------------------------------ */
 /* Interpolation helpers                                                    */
@@ -158,6 +163,9 @@ int orc_rebin(const struct recon_metadata *mr, const float *raw, float *out) {
      * (p_reshape / z_reshape / a_reshape, rebin_filter.cuh:99-109,173-183,231-245) */
     float *sheet = (float *)malloc(sizeof(float) * n_ffs * nrr * nch * n_proj);
     if (!sheet) return 2;
+    /* (the loops below write disjoint outputs: any thread order gives the
+     * same bits) */
+#pragma omp parallel for collapse(2) schedule(static)
     for (long fs = 0; fs < n_ffs; fs++)
         for (long r = 0; r < nrr; r++)
             for (long c = 0; c < nch; c++)
@@ -172,6 +180,7 @@ int orc_rebin(const struct recon_metadata *mr, const float *raw, float *out) {
         for (long r = 0; r < nrows; r++) {
             for (long p = 0; p < n_proj; p++)
                 for (long c = 0; c < nch; c++) rs[p * nch + c] = raw[(p * nrr + r) * nch + c];
+#pragma omp parallel for schedule(static)
             for (long ch = 0; ch < nos; ch++) {
                 REAL beta = RASIN(((REAL)ch - 2 * cc) * (dbeta / 2));
                 REAL beta_idx = beta / dbeta + cc;
@@ -204,6 +213,7 @@ int orc_rebin(const struct recon_metadata *mr, const float *raw, float *out) {
             for (long r = 0; r < nrr; r++) {
                 long orow = (fs == 0) ^ (cg->reverse_row_interleave != 0) ? 2 * r : 2 * r + 1;
                 const float *sh = sheet + (fs * nrr + r) * nch * n_proj;
+#pragma omp parallel for schedule(static)
                 for (long ch = 0; ch < nos; ch++) {
                     REAL beta = RASIN(((REAL)ch - 2 * cc) * (dbeta / 2) * (REAL)cg->r_f / rfr);
                     REAL beta_idx = orc_get_beta_idx(beta, l, (int)nch);
@@ -256,6 +266,7 @@ int orc_rebin(const struct recon_metadata *mr, const float *raw, float *out) {
             REAL rfr = n_ffs == 4 ? orc_r_fr(cg, 0, pr == 0 ? -dr : dr) : (REAL)cg->r_f;
             for (long r = 0; r < nrr; r++) {
                 /* pass 1 for this raw row: tsheet[m][p] */
+#pragma omp parallel for schedule(static)
                 for (long m = 0; m < nos; m++) {
                     int h = (int)(m & 1);
                     int fs = 2 * pr + h;
@@ -269,6 +280,7 @@ int orc_rebin(const struct recon_metadata *mr, const float *raw, float *out) {
                 long orow;
                 if (n_ffs == 4) orow = (pr == 0) ^ (cg->reverse_row_interleave != 0) ? 2 * r : 2 * r + 1;
                 else orow = r;
+#pragma omp parallel for schedule(static)
                 for (long ch = 0; ch < nos; ch++) {
                     REAL beta = RASIN(((REAL)ch - 2 * cc) * (dbeta / 2) * (REAL)cg->r_f / rfr);
                     REAL beta_idx = orc_get_beta_idx(beta, l, (int)nos);
@@ -379,7 +391,9 @@ int orc_backproject(const struct recon_metadata *mr, const float *rebin, float *
     const long nos = (long)cg->n_channels_oversampled, nrows = (long)cg->n_rows;
     const long half = (long)cg->n_proj_turn / 2;
     const long n_ht = (long)((mr->ri.n_proj_pull / mr->ri.n_ffs - 2 * cg->add_projections) / (cg->n_proj_turn / 2));
-    const int cpu = conventions == ORC_BP_CPU_CONVENTIONS;
+    const int cpu = (conventions & 1) == ORC_BP_CPU_CONVENTIONS;
+    const int no_flip = (conventions & ORC_BP_PLANT_NO_FLIP) != 0;
+    const int no_qsign = (conventions & ORC_BP_PLANT_NO_QSIGN) != 0;
     /* reference backproject.cu:37 (float tube_start, pi' constant) */
     float tube_start_f = (float)(fmod((double)(mr->tube_angles[0] +
                                                 ((float)(mr->ri.idx_pull_start + cg->add_projections_ffs) * 360.0f /
@@ -392,16 +406,24 @@ int orc_backproject(const struct recon_metadata *mr, const float *rebin, float *
     const REAL rf = (REAL)cg->r_f, ds = rf * (REAL)cg->fan_angle_increment / 2;
     const REAL tanc = RTAN((REAL)cg->theta_cone / 2);
     const REAL scale = 2 * pi / (REAL)cg->n_proj_turn;
+    long ncol = n_cols > 0 ? n_cols : nx * ny;
+    int fail = 0;
+    /* voxel columns are independent: one per iteration, any thread (each
+     * column's arithmetic is the same sequential statement) */
+#pragma omp parallel reduction(| : fail)
+    {
     REAL *s = (REAL *)malloc(sizeof(REAL) * nz);
     REAL *st = (REAL *)malloc(sizeof(REAL) * nz);
     REAL *ht = (REAL *)malloc(sizeof(REAL) * nz);
-    if (!s || !st || !ht) { free(s); free(st); free(ht); return 2; }
-    long ncol = n_cols > 0 ? n_cols : nx * ny;
+    if (!s || !st || !ht) fail = 1;
+#pragma omp for schedule(dynamic, 16)
     for (long ci = 0; ci < ncol; ci++) {
+        if (fail) continue;
         long xi = n_cols > 0 ? cols_x[ci] : ci / ny;
         long yi = n_cols > 0 ? cols_y[ci] : ci % ny;
         REAL xc = (REAL)xi - (REAL)(nx - 1) / 2;
-        REAL x = ((REAL)rp->recon_fov / (REAL)nx) * ((cpu || cg->table_direction == -1) ? xc : -xc) + (REAL)rp->x_origin;
+        REAL x = ((REAL)rp->recon_fov / (REAL)nx) * ((cpu || no_flip || cg->table_direction == -1) ? xc : -xc) +
+                 (REAL)rp->x_origin;
         REAL y = ((REAL)rp->recon_fov / (REAL)ny) * ((REAL)yi - (REAL)(ny - 1) / 2) + (REAL)rp->y_origin;
         for (long z = 0; z < nz; z++) s[z] = 0;
         for (long i = 0; i < half; i++) {
@@ -417,7 +439,7 @@ int orc_backproject(const struct recon_metadata *mr, const float *rebin, float *
                 for (long z = 0; z < nz; z++) {
                     REAL zz = (REAL)z * (REAL)rp->coll_slicewidth + (REAL)cg->z_rot / 2 +
                               (REAL)cg->z_rot * tube_start / (2 * pi);
-                    REAL qhat = (cpu ? 1 : -td) * (zz - ray) / (lhat * tanc);
+                    REAL qhat = ((cpu || no_qsign) ? 1 : -td) * (zz - ray) / (lhat * tanc);
                     REAL w = orc_W(qhat);
                     REAL val;
                     if (cpu) {
@@ -463,8 +485,9 @@ int orc_backproject(const struct recon_metadata *mr, const float *rebin, float *
     free(s);
     free(st);
     free(ht);
+    }
     (void)bp_fetch_cpu;
-    return 0;
+    return fail ? 2 : 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -634,6 +657,9 @@ void orc_adaptive_apply(const struct recon_metadata *mr, const float *raw, const
     const int nch = (int)mr->cg.n_channels, nrr = (int)mr->cg.n_rows_raw;
     const int np = (int)mr->ri.n_proj_pull;
     const long ps = (long)nch * nrr;
+    /* every output sample is its own sequential statement: any thread order
+     * gives the same bits */
+#pragma omp parallel for schedule(dynamic, 4)
     for (long p = 0; p < np; p++) {
         float T = thresholds[p];
         for (int i = 0; i < nch; i++)

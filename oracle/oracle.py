@@ -26,6 +26,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BP_GPU, BP_CPU = 0, 1
+# planted mistakes (added to BP_GPU) for tests that must see their checks fail
+BP_PLANT_NO_FLIP, BP_PLANT_NO_QSIGN = 2, 4
 
 _libs = {}
 
@@ -111,18 +113,29 @@ def filter_fft(rows, taps):
     CPU twin's method): out[l] = sum_k taps[N + l - k] rows[k]."""
     rows = np.asarray(rows, dtype=np.float64)
     n = rows.shape[-1]
-    m = 2 * int(2 ** np.ceil(np.log2(n)))
+    # any circular length M >= 2N - 1 gives the exact linear convolution (the
+    # reference uses M = 2 * 2^ceil(log2 N)); take a fast one
+    try:
+        from scipy.fft import next_fast_len
+        m = next_fast_len(2 * n - 1, real=True)
+    except ImportError:
+        m = 2 * int(2 ** np.ceil(np.log2(n)))
     f = np.zeros(m)
     taps = np.asarray(taps, dtype=np.float64)
     f[:n] = taps[n:2 * n]
     f[m - n + 1:] = taps[1:n]
-    F = np.fft.rfft(f)
+    try:
+        import scipy.fft as sfft       # row-parallel FFTs (each row on its own)
+        kw = {"workers": -1}
+    except ImportError:
+        sfft, kw = np.fft, {}
+    F = sfft.rfft(f)
     flat = rows.reshape(-1, n)
     out = np.empty_like(flat)
-    step = 4096
+    step = 8192
     for i in range(0, flat.shape[0], step):
-        R = np.fft.rfft(flat[i:i + step], n=m, axis=-1)
-        out[i:i + step] = np.fft.irfft(R * F, n=m, axis=-1)[:, :n]
+        R = sfft.rfft(flat[i:i + step], n=m, axis=-1, **kw)
+        out[i:i + step] = sfft.irfft(R * F, n=m, axis=-1, **kw)[:, :n]
     return out.reshape(rows.shape)
 
 
