@@ -390,3 +390,62 @@ def test_dicom_series_writer(tmp_path, mini_dir, table_dir, reverse):
     with pytest.raises(ValueError):
         cb.write_dicom_series(mr, stack[:-1], str(out), "ser")
     cb.free_metadata(mr)
+
+
+# Type 1 attributes of the CT Image IOD modules (patient/study/series/frame
+# of reference identifiers, general and CT image, image plane, image pixel,
+# SOP common) that the reference's CT Image Storage file holds and a
+# reconstructed series must hold too.
+CT_REQUIRED = ["0008,0016", "0008,0018", "0008,0060", "0008,0008", "0020,000d", "0020,000e", "0020,0013",
+               "0020,0032", "0020,0037", "0020,0052", "0018,0050", "0028,0002", "0028,0004", "0028,0010",
+               "0028,0011", "0028,0030", "0028,0100", "0028,0101", "0028,0102", "0028,0103", "0028,1052",
+               "0028,1053", "7fe0,0010"]
+
+
+def test_dicom_structure_matches_reference_ct_file(tmp_path, mini_dir):
+    """The writer's CT series against the structure of the CT Image Storage
+    file the reference ships (tests/golden/dicom/ct_image_structure.json,
+    written by tests/golden/make_dicom_structure.py from
+    resources/testfile.dcm): same transfer syntax (explicit VR little
+    endian) and SOP class, file-meta elements with the reference's VRs,
+    every data-set element the reference file also holds with the same VR,
+    the CT Image Type 1 attributes present, tags in ascending order, and the
+    same Image Pixel module (16-bit signed MONOCHROME2)."""
+    import json
+
+    import dicomfile as D
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "dicom", "ct_image_structure.json")))
+    ref_meta, ref_ds = dict(map(tuple, fx["meta"])), dict(map(tuple, fx["dataset"]))
+    assert all(k in ref_ds for k in CT_REQUIRED)
+    tok = scans.mini_tokens("z", nrows=8, nx=32, n_slices=3)
+    p = str(tmp_path / "s.prm")
+    cb.write_prm(p, tok)
+    mr = cb.metadata_from_prm(p, install_dir=mini_dir)
+    nf = cb.n_slices_final(mr)
+    stack = (np.arange(nf * 32 * 32) % 2000 - 1000).astype(np.int16).reshape(nf, 32, 32)
+    out = tmp_path / "dcm"
+    out.mkdir()
+    cb.write_dicom_series(mr, stack, str(out), "ser", slope=1.0, intercept=-1024.0)
+    for k in range(nf):
+        meta, data, order = D.read(str(out / f"ser_{k + 1:04d}.dcm"))
+        tags = [f"{g:04x},{e:04x}" for g, e in order]
+        assert order == sorted(order) and len(set(order)) == len(order)
+        assert D.text(meta[(0x2, 0x10)]) == fx["transfer_syntax"]
+        assert D.text(meta[(0x2, 0x2)]) == fx["media_storage_sop_class"]
+        assert D.text(data[(0x8, 0x16)]) == fx["sop_class"]
+        for (g, e), (vr, _) in meta.items():
+            assert ref_meta.get(f"{g:04x},{e:04x}") == vr, (g, e, vr)
+        assert set(ref_meta) <= {f"{g:04x},{e:04x}" for g, e in meta}
+        for (g, e), (vr, _) in data.items():
+            t = f"{g:04x},{e:04x}"
+            if t in ref_ds:
+                assert ref_ds[t] == vr, (t, vr, ref_ds[t])
+        assert all(t in tags for t in CT_REQUIRED), [t for t in CT_REQUIRED if t not in tags]
+        px = fx["pixel_module"]
+        assert D.us(data[(0x28, 0x2)]) == px["samples_per_pixel"]
+        assert D.us(data[(0x28, 0x100)]) == px["bits_allocated"]
+        assert D.us(data[(0x28, 0x101)]) == px["bits_stored"]
+        assert D.us(data[(0x28, 0x102)]) == px["high_bit"]
+        assert D.us(data[(0x28, 0x103)]) == px["pixel_representation"]
+        assert D.text(data[(0x28, 0x4)]) == px["photometric_interpretation"]
+    cb.free_metadata(mr)
