@@ -32,12 +32,14 @@ __all__ = [
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 FILTER_F32, FILTER_BF16X3, FILTER_F16X3, FILTER_FFT32 = 0, 1, 2, 3
 PROJ_F32, PROJ_BF16 = 0, 1
+NOALLOC_MAX_VIEWS = 16384      # CTBB_NOALLOC_MAX_VIEWS of include/ctbb_hip.h
 
 _STATUS = {
     0: "success", 1: "invalid argument", 2: "file missing, unreadable or unwritable",
     3: "essential prm value missing", 4: "requested reconstruction outside allowed data range",
     5: "could not parse selected scanner", 6: "filter file not found", 7: "geometry not supported by the kernels",
     8: "HIP runtime error", 9: "allocation failed", 10: "raw file type not supported by this build",
+    11: "input beyond the fixed capacity of a no-allocation entry point",
 }
 
 
@@ -93,6 +95,7 @@ _SIGS = {
     "ctbb_rebin_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_filter_rows_dev": (C.c_int, [_VP, _VP, C.c_int, C.c_size_t, _VP, _VP]),
     "ctbb_backproject_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
+    "ctbb_backproject_dev_noalloc": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_finalize_dev": (C.c_int, [_VP, _MR, _VP, _VP]),
     "ctbb_autotune_backprojection": (C.c_int, [_VP, _MR, C.c_char_p, _P(C.c_int)]),
     "ctbb_hu_to_int16_dev": (C.c_int, [_VP, _VP, C.c_size_t, C.c_float, C.c_float, _VP]),
@@ -305,6 +308,12 @@ class Context:
     def backproject_dev(self, mr, d_rebin, d_image_block):
         _check(lib().ctbb_backproject_dev(self._h, C.byref(mr), _ptr(d_rebin), _ptr(d_image_block)),
                "backproject_dev")
+
+    def backproject_dev_noalloc(self, mr, d_rebin, d_image_block):
+        """backproject_dev without allocating: the view table goes to a fixed
+        per-GPU device array (at most NOALLOC_MAX_VIEWS views per block)."""
+        _check(lib().ctbb_backproject_dev_noalloc(self._h, C.byref(mr), _ptr(d_rebin), _ptr(d_image_block)),
+               "backproject_dev_noalloc")
 
     def finalize_dev(self, mr, d_image, d_final):
         _check(lib().ctbb_finalize_dev(self._h, C.byref(mr), _ptr(d_image), _ptr(d_final)), "finalize_dev")

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,11 @@
 #ifndef CTBB_PIPE_DEPTH
 #define CTBB_PIPE_DEPTH 2
 #endif
+
+// View table of ctbb_backproject_dev_noalloc: a fixed device array, one copy
+// per GPU at its own address.  Host code never passes the array's name to a
+// kernel or a copy; it uses the per-device address hipGetSymbolAddress gives.
+static __device__ __align__(256) ctbb::BpView scratch[CTBB_NOALLOC_MAX_VIEWS];
 
 namespace {
 
@@ -50,12 +56,31 @@ struct EventPair {
     hipStream_t s;
 };
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on `device` (which must be current), looked up once
+// per device.
+int scratch_address(int device, ctbb::BpView **out) {
+    if (device < 0 || device >= kMaxDevices) return CTBB_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CTBB_ERR_HIP;
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<ctbb::BpView *>(g_scratch_ptr[device]);
+    return CTBB_OK;
+}
+
 }  // namespace
 
 struct ctbb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     ctbb_options opt{};
+    ctbb::BpView *noalloc_vtab = nullptr;   // this device's `scratch` array
     DevBuf raw_in, raw_af, rebin_tmp, image_block, image_stack, af_work;
     DevBuf vtab, maps, taps, spec, fin_tab;
     // host-streaming reconstruction: two raw pull buffers filled on a copy
@@ -344,7 +369,7 @@ int fill_bp_args(const recon_metadata *mr, BpArgs &a) {
 }
 
 int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin, const BpView *d_vtab,
-                   float *d_out, hipStream_t on = nullptr) {
+                   float *d_out, hipStream_t on = nullptr, bool timed = true) {
     if (!on) on = ctx->stream;
     BpArgs a{};
     int st = fill_bp_args(mr, a);
@@ -354,6 +379,7 @@ int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin,
     else a.rebin = static_cast<const float *>(d_rebin);
     a.vtab = d_vtab;
     a.out = d_out;
+    if (!timed) return launch_backproject(a, on);
     EventPair e = ctx->begin("backproject", on);
     st = launch_backproject(a, on);
     ctx->end(e);
@@ -604,6 +630,13 @@ int ctbb_ctx_create(int device, ctbb_ctx **out) {
         delete c;
         return CTBB_ERR_HIP;
     }
+    // the no-allocation entry point's array on this device (current here)
+    const int sst = scratch_address(c->device, &c->noalloc_vtab);
+    if (sst) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return sst;
+    }
     c->opt.filter_mode = CTBB_FILTER_FFT32;
     c->opt.proj_dtype = CTBB_PROJ_F32;
     c->opt.mu_water = 0.0f;
@@ -734,6 +767,23 @@ int ctbb_backproject_dev(ctbb_ctx *c, const struct recon_metadata *mr, const voi
     if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
     c->reset_events();
     return backproject_dev(c, mr, d_rebin, d_image_block);
+}
+
+// Same as ctbb_backproject_dev with the view table in the device array
+// `scratch` (through this device's address, looked up at context creation):
+// no allocation, no event timing.
+int ctbb_backproject_dev_noalloc(ctbb_ctx *c, const struct recon_metadata *mr, const void *d_rebin,
+                                 float *d_image_block) {
+    if (!c || !mr || !d_rebin || !d_image_block || !mr->tube_angles || !c->noalloc_vtab) return CTBB_ERR_ARG;
+    int st = check_geometry(mr);
+    if (st) return st;
+    const VtabJob j = vtab_job(*mr, 0);
+    if (j.n_ht <= 0 || j.half <= 0) return CTBB_ERR_SHAPE;
+    if (vtab_len(j) > static_cast<size_t>(CTBB_NOALLOC_MAX_VIEWS)) return CTBB_ERR_LIMIT;
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
+    st = launch_view_table(c->noalloc_vtab, j.tube_start, j.half, j.n_ht, j.n_proj_turn, j.z_rot, c->stream);
+    if (st) return st;
+    return do_backproject(c, mr, d_rebin, c->noalloc_vtab, d_image_block, c->stream, false);
 }
 
 // Backprojection autotuner: bp_tile 16 (two z-groups) or 32 (one group).

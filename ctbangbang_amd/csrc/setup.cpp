@@ -150,6 +150,7 @@ const char *ctbb_strerror(int s) {
         case CTBB_ERR_HIP: return "HIP runtime error";
         case CTBB_ERR_NOMEM: return "allocation failed";
         case CTBB_ERR_UNSUPPORTED: return "raw file type not supported by this build";
+        case CTBB_ERR_LIMIT: return "input beyond the fixed capacity of a no-allocation entry point";
     }
     return "unknown status";
 }

@@ -44,7 +44,8 @@ typedef enum {
     CTBB_ERR_SHAPE = 7,     /* geometry outside what the kernels support */
     CTBB_ERR_HIP = 8,       /* HIP runtime error (no device, launch failure) */
     CTBB_ERR_NOMEM = 9,     /* host or device allocation failed */
-    CTBB_ERR_UNSUPPORTED = 10 /* raw file type without a reader in this build */
+    CTBB_ERR_UNSUPPORTED = 10, /* raw file type without a reader in this build */
+    CTBB_ERR_LIMIT = 11       /* input beyond the fixed capacity of a no-allocation entry point */
 } ctbb_status;
 
 const char *ctbb_strerror(int status);
@@ -220,6 +221,20 @@ int ctbb_filter_rows_dev(ctbb_ctx *ctx, const float *d_taps, int n, size_t n_row
 /* Backproject a filtered block into d_image_block [32][nx][ny] (overwritten). */
 int ctbb_backproject_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
                          const void *d_rebin, float *d_image_block);
+
+/* ctbb_backproject_dev without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * block's view table (its only scratch) goes to a fixed device array of
+ * CTBB_NOALLOC_MAX_VIEWS entries (256 KiB per GPU, held until the process
+ * ends) instead of the context's workspace; views = (n_proj_turn / 2) *
+ * half turns of the block beyond that limit return CTBB_ERR_LIMIT before
+ * anything is launched.  The array is shared by every call on a GPU: calls
+ * that overlap on one GPU (other contexts or streams) race.  No event
+ * timing is recorded.  Create the context (ctbb_ctx_create) beforehand. */
+#define CTBB_NOALLOC_MAX_VIEWS 16384
+int ctbb_backproject_dev_noalloc(ctbb_ctx *ctx, const struct recon_metadata *mr,
+                                 const void *d_rebin, float *d_image_block);
 
 /* Reorder + thicken (+HU) d_image [n_slices_recon][nx][ny] into d_final. */
 int ctbb_finalize_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
