@@ -40,6 +40,7 @@ _STATUS = {
     5: "could not parse selected scanner", 6: "filter file not found", 7: "geometry not supported by the kernels",
     8: "HIP runtime error", 9: "allocation failed", 10: "raw file type not supported by this build",
     11: "input beyond the fixed capacity of a no-allocation entry point",
+    12: "raw data holds NaN or infinite samples",
 }
 
 
@@ -66,6 +67,7 @@ _SIGS = {
     "ctbb_empty_config": (C.c_int, [C.c_char_p]),
     "ctbb_configure_recon_params": (C.c_int, [C.c_char_p, _P(ReconParams)]),
     "ctbb_prm_mu_water": (C.c_int, [C.c_char_p, _P(C.c_float)]),
+    "ctbb_prm_float": (C.c_int, [C.c_char_p, C.c_char_p, _P(C.c_float)]),
     "ctbb_configure_paths": (C.c_int, [_MR]),
     "ctbb_configure_ct_geom": (C.c_int, [_MR, _P(CtGeom)]),
     "ctbb_configure_reconstruction": (C.c_int, [_MR]),

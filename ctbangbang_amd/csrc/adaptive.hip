@@ -14,6 +14,9 @@
 //      the view's window copied once to LDS
 //   5. 5x5x5 triangle smoothing of samples above threshold: one thread per
 //      sample (coalesced along channels)
+// A fixed-threshold mode (no counterpart in the reference) skips steps 1-4
+// and smooths the samples above one given line-integral threshold, e.g.
+// ln(I0 / N) for a noise threshold of N photons at I0 incident photons.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -215,6 +218,12 @@ __global__ __launch_bounds__(256) void af_apply4(const float *raw, const float *
     *reinterpret_cast<float4 *>(out + base) = y;
 }
 
+// Fixed-threshold mode: every view's threshold is the same constant.
+__global__ __launch_bounds__(256) void af_const(float *thr, int n_proj, float T) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n_proj) thr[p] = T;
+}
+
 int odd_window(float frac, int n_proj_ffs) {
     int n = static_cast<int>(ceilf(frac * static_cast<float>(n_proj_ffs)));
     return n + (1 - n % 2);
@@ -233,17 +242,22 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     const int n_avg = odd_window(18.0f / 360.0f, a.n_proj_ffs);
     const int n_range = odd_window(180.0f / 360.0f, a.n_proj_ffs);
     const unsigned g1 = static_cast<unsigned>((a.n_proj + 255) / 256);
-    hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
-    hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
-    hipLaunchKernelGGL(af_ecc, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, a.n_proj, n_range);
-    // the counting window (at most n_range values) in LDS when it takes <= 16 KB
-    const size_t win_bytes = static_cast<size_t>(n_range) * sizeof(float);
-    if (win_bytes <= 16384)
-        hipLaunchKernelGGL(af_thresh<true>, dim3(a.n_proj), dim3(64), win_bytes, stream, sm, mx, mn, ecc, thr,
-                           a.n_proj, n_range, a.n_proj_ffs, a.strength);
-    else
-        hipLaunchKernelGGL(af_thresh<false>, dim3(a.n_proj), dim3(64), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
-                           n_range, a.n_proj_ffs, a.strength);
+    if (a.fixed_threshold > 0.0f) {
+        if (!(a.fixed_threshold < 3.0e38f)) return CTBB_ERR_ARG;
+        hipLaunchKernelGGL(af_const, dim3(g1), dim3(256), 0, stream, thr, a.n_proj, a.fixed_threshold);
+    } else {
+        hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
+        hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
+        hipLaunchKernelGGL(af_ecc, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, a.n_proj, n_range);
+        // the counting window (at most n_range values) in LDS when it takes <= 16 KB
+        const size_t win_bytes = static_cast<size_t>(n_range) * sizeof(float);
+        if (win_bytes <= 16384)
+            hipLaunchKernelGGL(af_thresh<true>, dim3(a.n_proj), dim3(64), win_bytes, stream, sm, mx, mn, ecc, thr,
+                               a.n_proj, n_range, a.n_proj_ffs, a.strength);
+        else
+            hipLaunchKernelGGL(af_thresh<false>, dim3(a.n_proj), dim3(64), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
+                               n_range, a.n_proj_ffs, a.strength);
+    }
     const bool aligned = (reinterpret_cast<uintptr_t>(a.raw) & 15u) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 15u) == 0;
     if (a.vec4 && aligned)
         hipLaunchKernelGGL(af_apply4, dim3(static_cast<unsigned>((frame / 4 + 255) / 256), a.n_proj), dim3(256), 0,

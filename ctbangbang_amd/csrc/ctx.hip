@@ -83,6 +83,9 @@ struct ctbb_ctx {
     ctbb::BpView *noalloc_vtab = nullptr;   // this device's `scratch` array
     DevBuf raw_in, raw_af, rebin_tmp, image_block, image_stack, af_work;
     DevBuf vtab, maps, taps, spec, fin_tab;
+    // 1 once a rebin met a non-finite output sample (read and cleared by
+    // ctbb_ctx_synchronize)
+    DevBuf nonfinite;
     // host-streaming reconstruction: two raw pull buffers filled on a copy
     // stream (default priority) while the compute stream works on the other
     DevBuf pull[2];
@@ -303,6 +306,7 @@ int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float 
     a.n_proj_out = a.n_proj - 2 * a.add;
     if (n_views_max >= 0 && n_views_max < a.n_proj_out) a.n_proj_out = n_views_max;
     a.reverse_rows = cg.reverse_row_interleave;
+    a.nonfinite = ctx->nonfinite.as<unsigned>();
     EventPair e = ctx->begin("rebin");
     st = launch_rebin(a, ctx->stream);
     ctx->end(e);
@@ -485,6 +489,7 @@ int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, flo
     a.n_proj_ffs = static_cast<int>(cg.n_proj_ffs);
     a.strength = mr->rp.adaptive_filtration_s;
     a.vec4 = vec4 ? 1 : 0;
+    a.fixed_threshold = ctx->opt.adaptive_threshold;
     int st = ctx->af_work.ensure(6 * static_cast<size_t>(a.n_proj) * sizeof(float));
     if (st) return st;
     a.work = ctx->af_work.as<float>();
@@ -605,6 +610,16 @@ int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int item, int b, const float
     return hip_ok(hipEventRecord(c->ev_bp[k], c->stream_bp));
 }
 
+// After the context's work has finished: CTBB_ERR_NONFINITE (and the flag
+// cleared) when a rebin met a non-finite sample since the last check.
+int take_nonfinite(ctbb_ctx *c) {
+    unsigned flag = 0;
+    if (hipMemcpy(&flag, c->nonfinite.p, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess) return CTBB_ERR_HIP;
+    if (!flag) return CTBB_OK;
+    if (hipMemset(c->nonfinite.p, 0, sizeof(flag)) != hipSuccess) return CTBB_ERR_HIP;
+    return CTBB_ERR_NONFINITE;
+}
+
 // ctx->stream waits for every backprojection queued on stream_bp
 int join_pipeline(ctbb_ctx *c) {
     if (hipEventRecord(c->ev_sync, c->stream_bp) != hipSuccess) return CTBB_ERR_HIP;
@@ -631,8 +646,11 @@ int ctbb_ctx_create(int device, ctbb_ctx **out) {
         return CTBB_ERR_HIP;
     }
     // the no-allocation entry point's array on this device (current here)
-    const int sst = scratch_address(c->device, &c->noalloc_vtab);
+    int sst = scratch_address(c->device, &c->noalloc_vtab);
+    if (!sst) sst = c->nonfinite.ensure(sizeof(unsigned));
+    if (!sst && hipMemsetAsync(c->nonfinite.p, 0, sizeof(unsigned), c->stream) != hipSuccess) sst = CTBB_ERR_HIP;
     if (sst) {
+        c->nonfinite.release();
         (void)hipStreamDestroy(c->stream);
         delete c;
         return sst;
@@ -655,7 +673,7 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     if (c->stream_bp) (void)hipStreamSynchronize(c->stream_bp);
     for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->image_block, &c->image_stack, &c->af_work,
                       &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1],
-                      &c->image_stack2})
+                      &c->image_stack2, &c->nonfinite})
         b->release();
     for (DevBuf &b : c->filt) b.release();
     for (int i = 0; i < 2; i++) {
@@ -690,6 +708,7 @@ int ctbb_ctx_set_options(ctbb_ctx *c, const ctbb_options *o) {
     if (o->proj_dtype != CTBB_PROJ_F32 && o->proj_dtype != CTBB_PROJ_BF16) return CTBB_ERR_ARG;
     if (o->bp_tile != 0 && o->bp_tile != 16 && o->bp_tile != 32) return CTBB_ERR_ARG;
     if (o->bp_views_per_pass != 0 && o->bp_views_per_pass != 1) return CTBB_ERR_ARG;
+    if (!(o->adaptive_threshold >= 0.0f && o->adaptive_threshold < 3.0e38f)) return CTBB_ERR_ARG;
     c->opt = *o;
     return CTBB_OK;
 }
@@ -699,12 +718,16 @@ void *ctbb_ctx_stream(ctbb_ctx *c) { return c ? static_cast<void *>(c->stream) :
 int ctbb_ctx_synchronize(ctbb_ctx *c) {
     if (!c) return CTBB_ERR_ARG;
     if (c->stream_bp && hipStreamSynchronize(c->stream_bp) != hipSuccess) return CTBB_ERR_HIP;
-    return hip_ok(hipStreamSynchronize(c->stream));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return CTBB_ERR_HIP;
+    return take_nonfinite(c);
 }
 
 int ctbb_ctx_kernel_time(ctbb_ctx *c, const char *family, float *total_ms, int *launches) {
     if (!c || !family || !total_ms || !launches) return CTBB_ERR_ARG;
-    if (ctbb_ctx_synchronize(c) != CTBB_OK) return CTBB_ERR_HIP;
+    // (the streams only: a pending non-finite report stays for synchronize)
+    if ((c->stream_bp && hipStreamSynchronize(c->stream_bp) != hipSuccess) ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
     *total_ms = 0.0f;
     *launches = 0;
     if (c->timing_failed) return CTBB_ERR_HIP;
@@ -1084,7 +1107,8 @@ int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h
     if (hipMemcpyAsync(h_final, c->raw_in.p, final_bytes, hipMemcpyDeviceToHost,
                        c->stream) != hipSuccess)
         return CTBB_ERR_HIP;
-    return hip_ok(hipStreamSynchronize(c->stream));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return CTBB_ERR_HIP;
+    return take_nonfinite(c);
 }
 
 int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *const *mrs,
@@ -1248,6 +1272,10 @@ int ctbb_rebin_filter(ctbb_ctx *c, struct recon_metadata *mr) {
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         (void)hipFree(d);
         return CTBB_ERR_HIP;
+    }
+    if ((st = take_nonfinite(c))) {
+        (void)hipFree(d);
+        return st;
     }
     mr->ctd.d_rebin = static_cast<float *>(d);
     return CTBB_OK;

@@ -43,6 +43,7 @@ struct RebinArgs {
     int kind;                  // 0 nffs, 1 zffs, 2 pffs, 3 affs
     int n_ffs, nch, nos, n_rows_raw, n_rows, n_proj, n_proj_out, add;
     int reverse_rows;
+    unsigned *nonfinite;       // set to 1 when an output sample is not finite (or null)
 };
 int launch_rebin(const RebinArgs &a, hipStream_t stream);
 
@@ -70,6 +71,7 @@ struct AdaptiveArgs {
     int n_proj_ffs;
     float strength;
     int vec4;                  // 1: 16-byte apply kernel where aligned (GPU otherwise idle)
+    float fixed_threshold;     // > 0: this threshold for every view instead of the KK bisection
 };
 int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream);
 

@@ -90,9 +90,12 @@ int main(int argc, char **argv) {
     opt.filter_mode = CTBB_FILTER_F32;
     opt.proj_dtype = CTBB_PROJ_F32;
     opt.adaptive_filter = 1;
-    float mu_w = 0.0f;
+    float mu_w = 0.0f, thr = 0.0f;
     ctbb_prm_mu_water(prm, &mu_w);
     opt.mu_water = mu_w;
+    // fixed adaptive-filter threshold (line integral), a token of this build
+    ctbb_prm_float(prm, "AdaptiveThreshold:", &thr);
+    opt.adaptive_threshold = thr > 0.0f ? thr : 0.0f;
     ctbb_compat_set_options(&opt);
 
     logv("Configuring scanner geometry...\n");

@@ -17,6 +17,12 @@
 // Output element (v, r, c) = rebinned projection v + add_projections, row r,
 // oversampled channel c.  Threads run along c (coalesced writes, raw reads at
 // channel ~c/2 of a handful of readings that stay in L1/L2).
+//
+// Every raw sample the reconstruction uses reaches some rebinned sample
+// (non-finite values propagate through the interpolation), so the kernels
+// also report non-finite input: a wave with a non-finite output sample
+// stores 1 to *nonfinite (the reference instead drops such fetches in its
+// backprojection and resets h_t, include/backproject.cuh:89-95).
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -35,6 +41,14 @@ __device__ __forceinline__ void taps(float x, int n, int &i0, int &i1, float &f)
     i0 = static_cast<int>(fl);
     i1 = min(i0 + 1, n - 1);
     f = x - fl;
+}
+
+// One plain store of 1 per wave that saw a non-finite output sample (every
+// writer stores the same value, so no atomic is needed).
+__device__ __forceinline__ void report_nonfinite(unsigned *flag, bool bad) {
+    const unsigned long long any = __builtin_amdgcn_ballot_w64(bad);
+    if (any && flag && (threadIdx.x & 63) == static_cast<unsigned>(__builtin_ctzll(any)))
+        *reinterpret_cast<volatile unsigned *>(flag) = 1u;
 }
 
 __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
@@ -70,6 +84,7 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
         ao1 = aoff[set * a.nos + m1];
     }
 
+    bool bad = false;
     for (int vv = 0; vv < kViewsPerThread; vv++) {
         const int v = blockIdx.z * kViewsPerThread + vv;
         if (v >= a.n_proj_out) break;
@@ -107,7 +122,9 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
             out = fmaf(fm, t[1] - t[0], t[0]);
         }
         outp[(static_cast<size_t>(v) * a.n_rows + r) * a.nos + c] = out;
+        bad |= !__builtin_isfinite(out);
     }
+    report_nonfinite(a.nonfinite, bad);
 }
 
 // The same arithmetic with 32-bit byte offsets from the (uniform) raw and
@@ -149,6 +166,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void r
     const int v0 = blockIdx.z * kViewsPerThread;
     unsigned ob = (static_cast<unsigned>(v0 * a.n_rows + r) * static_cast<unsigned>(a.nos) + c) * 4u;
     const unsigned ostep = static_cast<unsigned>(a.n_rows * a.nos) * 4u;
+    bool bad = false;
     for (int vv = 0; vv < kViewsPerThread; vv++) {
         const int v = v0 + vv;
         if (v >= a.n_proj_out) break;
@@ -183,8 +201,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void r
             out = fmaf(fc, t[1] - t[0], t[0]);
         }
         *reinterpret_cast<float *>(reinterpret_cast<char *>(a.out) + ob) = out;
+        bad |= !__builtin_isfinite(out);
         ob += ostep;
     }
+    report_nonfinite(a.nonfinite, bad);
 }
 
 }  // namespace

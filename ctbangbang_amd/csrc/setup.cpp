@@ -151,6 +151,7 @@ const char *ctbb_strerror(int s) {
         case CTBB_ERR_NOMEM: return "allocation failed";
         case CTBB_ERR_UNSUPPORTED: return "raw file type not supported by this build";
         case CTBB_ERR_LIMIT: return "input beyond the fixed capacity of a no-allocation entry point";
+        case CTBB_ERR_NONFINITE: return "raw data holds NaN or infinite samples";
     }
     return "unknown status";
 }
@@ -193,23 +194,27 @@ int ctbb_empty_config(const char *filepath) {
     return CTBB_OK;
 }
 
-int ctbb_prm_mu_water(const char *prm_file, float *mu_water) {
-    if (!prm_file || !mu_water) return CTBB_ERR_ARG;
-    *mu_water = 0.0f;
+int ctbb_prm_float(const char *prm_file, const char *token, float *value) {
+    if (!prm_file || !token || !value) return CTBB_ERR_ARG;
+    *value = 0.0f;
     FILE *f = std::fopen(prm_file, "r");
     if (!f) return CTBB_ERR_IO;
     char raw_line[1024];
     while (std::fgets(raw_line, sizeof(raw_line), f)) {
         raw_line[std::strcspn(raw_line, "%")] = '\0';
         char *save = nullptr;
-        char *tok = strtok_r(raw_line, " \t", &save);
-        if (tok && std::strcmp(tok, "MuWater:") == 0) {
-            tok = strtok_r(nullptr, " \t", &save);
-            if (tok) std::sscanf(tok, "%f", mu_water);
+        char *tok = strtok_r(raw_line, " \t\r\n", &save);
+        if (tok && std::strcmp(tok, token) == 0) {
+            tok = strtok_r(nullptr, " \t\r\n", &save);
+            if (tok) std::sscanf(tok, "%f", value);
         }
     }
     std::fclose(f);
     return CTBB_OK;
+}
+
+int ctbb_prm_mu_water(const char *prm_file, float *mu_water) {
+    return ctbb_prm_float(prm_file, "MuWater:", mu_water);
 }
 
 // reference src/setup.cu:108-176

@@ -153,6 +153,7 @@ class Options(C.Structure):
         ("bp_tile", C.c_int),
         ("bp_views_per_pass", C.c_int),
         ("adaptive_filter", C.c_int),
+        ("adaptive_threshold", C.c_float),
     ]
 
 

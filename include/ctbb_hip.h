@@ -45,7 +45,8 @@ typedef enum {
     CTBB_ERR_HIP = 8,       /* HIP runtime error (no device, launch failure) */
     CTBB_ERR_NOMEM = 9,     /* host or device allocation failed */
     CTBB_ERR_UNSUPPORTED = 10, /* raw file type without a reader in this build */
-    CTBB_ERR_LIMIT = 11       /* input beyond the fixed capacity of a no-allocation entry point */
+    CTBB_ERR_LIMIT = 11,      /* input beyond the fixed capacity of a no-allocation entry point */
+    CTBB_ERR_NONFINITE = 12   /* the raw data reaching the reconstruction held NaN or infinite samples */
 } ctbb_status;
 
 const char *ctbb_strerror(int status);
@@ -63,6 +64,10 @@ int ctbb_configure_recon_params(const char *prm_file, struct recon_params *out);
 /* Linear attenuation of water (1/mm) given by the "MuWater:" token of a prm
  * file, or 0 when absent.  The token is unknown to the reference parser. */
 int ctbb_prm_mu_water(const char *prm_file, float *mu_water);
+
+/* Value of a numeric prm token this build adds (e.g. "AdaptiveThreshold:"),
+ * or 0 when absent; the reference parser ignores such tokens. */
+int ctbb_prm_float(const char *prm_file, const char *token, float *value);
 
 /* Resolve cwd / install_dir / output path and check the raw file is readable
  * (1) and the output writable (2): reference src/setup.cu:40-106.
@@ -141,6 +146,11 @@ typedef struct ctbb_options_s {
                                    workgroup; 0 selects it) or 32; other values are rejected */
     int bp_views_per_pass;      /* reserved: the kernels stage one view per LDS pass (0 or 1) */
     int adaptive_filter;        /* 1 = run the adaptive raw-data filter (default) */
+    float adaptive_threshold;   /* 0: the reference's per-view threshold (Kachelriess-Kalender
+                                   bisection); > 0: smooth every sample above this fixed
+                                   line-integral threshold, e.g. ln(I0 / N) for a noise threshold
+                                   of N photons at I0 incident photons (no counterpart in the
+                                   reference) */
 } ctbb_options;
 
 /* Create a context on HIP device `device` (-1: current device).  The context
@@ -151,7 +161,11 @@ int ctbb_ctx_get_options(const ctbb_ctx *ctx, ctbb_options *opt);
 int ctbb_ctx_set_options(ctbb_ctx *ctx, const ctbb_options *opt);
 /* The HIP stream (hipStream_t) the context launches on, as an opaque pointer. */
 void *ctbb_ctx_stream(ctbb_ctx *ctx);
-/* Block until all work queued on the context's stream has finished. */
+/* Block until all work queued on the context's streams has finished.
+ * Returns CTBB_ERR_NONFINITE (once) when a rebin since the previous call
+ * met a non-finite raw sample: the results of that work are undefined.  The
+ * reference's backprojection drops such samples and resets h_t
+ * (include/backproject.cuh:89-95); this build reports them instead. */
 int ctbb_ctx_synchronize(ctbb_ctx *ctx);
 /* Device time of a kernel family during the most recent top-level call on
  * the context, from HIP events recorded on the context stream around each

@@ -449,3 +449,27 @@ def test_dicom_structure_matches_reference_ct_file(tmp_path, mini_dir):
         assert D.us(data[(0x28, 0x103)]) == px["pixel_representation"]
         assert D.text(data[(0x28, 0x4)]) == px["photometric_interpretation"]
     cb.free_metadata(mr)
+
+
+def test_build_side_prm_tokens(tmp_path):
+    """Tokens this build adds to the prm (MuWater, AdaptiveThreshold) are
+    read by ctbb_prm_float / ctbb_prm_mu_water and ignored by the reference
+    parser's grammar (comments after %, any whitespace)."""
+    p = tmp_path / "t.prm"
+    p.write_text("MuWater:\t0.0193 % water\nAdaptiveThreshold:  1.2040 \nNx: 64\n")
+    v = C.c_float()
+    L = cb.lib()
+    assert L.ctbb_prm_float(str(p).encode(), b"AdaptiveThreshold:", C.byref(v)) == 0
+    assert abs(v.value - 1.204) < 1e-6
+    assert L.ctbb_prm_mu_water(str(p).encode(), C.byref(v)) == 0
+    assert abs(v.value - 0.0193) < 1e-7
+    assert L.ctbb_prm_float(str(p).encode(), b"Missing:", C.byref(v)) == 0 and v.value == 0.0
+    assert L.ctbb_prm_float(str(tmp_path / "none.prm").encode(), b"MuWater:", C.byref(v)) == 2
+
+
+def test_status_strings():
+    L = cb.lib()
+    assert L.ctbb_strerror(12) == b"raw data holds NaN or infinite samples"
+    assert L.ctbb_strerror(11).startswith(b"input beyond the fixed capacity")
+    for code in range(13):
+        assert cb.CTBBError(code).code == code and "unknown" not in str(cb.CTBBError(code))

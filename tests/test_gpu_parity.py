@@ -713,3 +713,115 @@ def test_context_options_validated(ctx):
             (before.bp_tile, before.bp_views_per_pass, before.filter_mode, before.proj_dtype)
     ctx.set_options(bp_tile=32)
     ctx.set_options(bp_tile=0)
+
+
+def test_nonfinite_raw_is_reported(ctx, tmp_path, mini_dir):
+    """A NaN or infinite raw sample that enters the reconstruction makes the
+    next synchronize report CTBB_ERR_NONFINITE (12) once; the host-streaming
+    and host-contract entry points return it themselves.  A finite scan
+    afterwards reconstructs cleanly and to the same bits as before."""
+    tok = scans.mini_tokens("both", nrows=8, nx=32)
+    mr = _mr(tmp_path, tok, mini_dir)
+    n_read = tok["Readings"]
+    frame = mr.cg.n_rows_raw * mr.cg.n_channels
+    raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda")
+    ctx.synth_scan_dev(mr, scans.phantom_for(tok, 2), 0, n_read, raw)
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, adaptive_threshold=0.0, filter_mode=cb.FILTER_FFT32,
+                    proj_dtype=cb.PROJ_F32, bp_tile=0)
+    s = cb.block_sizes(mr)
+    out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.synchronize()
+    ctx.reconstruct_dev(mr, raw, n_read, out)
+    ctx.synchronize()
+    want = host(out)
+    cb.plan_block(mr, 0)
+    mid = int(mr.ri.idx_pull_start) + int(mr.ri.n_proj_pull) // 2
+    for bad in (float("nan"), float("inf")):
+        r = raw.clone()
+        r[mid * frame + 3 * mr.cg.n_channels + 60] = bad
+        torch.cuda.synchronize()
+        ctx.reconstruct_dev(mr, r, n_read, out)
+        with pytest.raises(cb.CTBBError) as e:
+            ctx.synchronize()
+        assert e.value.code == 12
+        ctx.synchronize()                       # reported once
+        h = r.cpu().pin_memory()
+        h_out = torch.empty(s.final_stack, dtype=torch.float32).pin_memory()
+        with pytest.raises(cb.CTBBError) as e:
+            ctx.reconstruct_host(mr, h, n_read, h_out)
+        assert e.value.code == 12
+    ctx.reconstruct_dev(mr, raw, n_read, out)
+    ctx.synchronize()
+    assert np.array_equal(host(out), want)
+    cb.free_metadata(mr)
+
+
+@pytest.mark.parametrize("ffs", ["none", "both"])
+def test_adaptive_fixed_threshold_bit_exact(ctx, tmp_path, mini_dir, ffs):
+    """Fixed-threshold adaptive filter (every view smooths the samples above
+    one line-integral threshold) bit for bit against the restatement's
+    smoothing step with that threshold (stage entry point: the 16-byte apply
+    kernel)."""
+    import oracle
+    tok = scans.mini_tokens(ffs, nrows=8 if ffs == "both" else 4)
+    mr = _mr(tmp_path, tok, mini_dir)
+    cb.plan_block(mr, 0)
+    raw = _raw_pull(ctx, mr, scans.phantom_for(tok, 4))
+    r = host(raw)
+    T = float(np.float32(np.quantile(r, 0.7)))
+    ctx.set_options(adaptive_threshold=T)
+    out = torch.empty_like(raw)
+    ctx.adaptive_filter_dev(mr, raw, out)
+    ctx.synchronize()
+    ctx.set_options(adaptive_threshold=0.0)
+    n = int(mr.ri.n_proj_pull)
+    want = oracle.adaptive_apply(mr, r.reshape(n, -1), np.full(n, np.float32(T), dtype=np.float32))
+    got = host(out).reshape(want.shape)
+    changed = int(np.count_nonzero(want != r.reshape(want.shape)))
+    print(f"adaptive fixed T={T:.4f} ({ffs}): {changed} samples smoothed, mismatches {np.count_nonzero(got != want)}")
+    assert changed > 0.2 * want.size
+    assert np.array_equal(got, want)
+    cb.free_metadata(mr)
+
+
+def test_end_to_end_fixed_threshold(ctx, tmp_path, mini_dir):
+    """Whole-scan pipeline with the fixed-threshold adaptive filter (blocks
+    after the first use the one-sample apply kernel beside the running
+    backprojection) against the FP64 oracle with the same threshold."""
+    import oracle
+    tok = scans.mini_tokens("both", nrows=8, nx=32, rotations=24, n_slices=40)
+    mr = _mr(tmp_path, tok, mini_dir)
+    assert int(mr.ri.n_blocks) >= 3
+    n_read = tok["Readings"]
+    frame = mr.cg.n_rows_raw * mr.cg.n_channels
+    raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda")
+    ctx.synth_scan_dev(mr, scans.phantom_for(tok, 6), 0, n_read, raw)
+    raw_h = host(raw).reshape(n_read, frame)
+    T = float(np.float32(np.log(1e5 / 3e4)))
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, adaptive_threshold=T, filter_mode=cb.FILTER_FFT32,
+                    proj_dtype=cb.PROJ_F32, bp_tile=0)
+    s = cb.block_sizes(mr)
+    out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.reconstruct_dev(mr, raw, n_read, out)
+    ctx.synchronize()
+    ctx.set_options(adaptive_threshold=0.0)
+    got = host(out).reshape(-1, mr.rp.nx, mr.rp.ny)
+    taps = cb.load_filter(mr)
+    blocks, smoothed = [], 0
+    for b in range(mr.ri.n_blocks):
+        cb.plan_block(mr, b)
+        s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+        win = np.zeros((n, frame), dtype=np.float32)
+        lo, hi = max(s0, 0), min(s0 + n, n_read)
+        win[lo - s0:hi - s0] = raw_h[lo:hi]
+        af = oracle.adaptive_apply(mr, win, np.full(n, np.float32(T), dtype=np.float32))
+        smoothed += int(np.count_nonzero(af != win))
+        blocks.append(oracle.backproject(mr, oracle.rebin_filter(mr, af, taps, double=True), oracle.BP_GPU,
+                                         double=True))
+    want = oracle.finalize(mr, np.concatenate(blocks), 0.02, double=True)
+    d = got.astype(np.float64) - want
+    rms, mx = float(np.sqrt(np.mean(d ** 2))), float(np.abs(d).max())
+    print(f"end-to-end fixed threshold {T:.4f}: {smoothed} samples smoothed; HU rms {rms:.4f} max {mx:.4f}")
+    assert smoothed > 0
+    assert rms <= 0.05 and mx <= 2.0
+    cb.free_metadata(mr)

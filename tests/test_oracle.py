@@ -336,3 +336,28 @@ def test_adaptive_threshold_property(tmp_path, mini_dir, eccentric):
     # samples at or below their view's threshold are untouched
     below = raw <= T[:, None, None]
     assert np.array_equal(out[below], raw[below])
+
+
+@pytest.mark.parametrize("ffs", ["none", "both"])
+def test_nonfinite_raw_sample_reaches_rebinned_sinogram(mini_dir, tmp_path, ffs):
+    """A NaN or infinite raw sample inside the pull shows up as non-finite
+    values of the rebinned sinogram (every raw sample the reconstruction
+    uses is an interpolation tap of some rebinned sample), which is where the
+    HIP rebin checks its output; a finite pull gives a finite sinogram."""
+    tok = scans.mini_tokens(ffs, nrows=8 if ffs == "both" else 4)
+    p = str(tmp_path / "n.prm")
+    cb.write_prm(p, tok)
+    mr = cb.metadata_from_prm(p, install_dir=mini_dir)
+    cb.plan_block(mr, 0)
+    n = int(mr.ri.n_proj_pull)
+    raw = np.ones((n, mr.cg.n_rows_raw, mr.cg.n_channels), dtype=np.float32)
+    assert np.isfinite(oracle.rebin(mr, raw, double=False)).all()
+    for bad in (np.nan, np.inf):
+        r = raw.copy()
+        r[n // 2, 1, 40] = bad
+        out = oracle.rebin(mr, r, double=False)
+        assert not np.isfinite(out).all()
+        # only views near the bad reading are affected
+        views = np.unique(np.nonzero(~np.isfinite(out))[0])
+        assert views.size < 8 * int(mr.ri.n_ffs)
+    cb.free_metadata(mr)
