@@ -106,6 +106,9 @@ _SIGS = {
     "ctbb_reconstruct_host": (C.c_int, [_VP, _MR, _VP, C.c_size_t, _VP]),
     "ctbb_reconstruct_batch_dev": (C.c_int, [_VP, C.c_int, C.POINTER(_MR), C.POINTER(_VP), C.POINTER(C.c_size_t),
                                              C.POINTER(_VP)]),
+    "ctbb_rebin_filter_cpu": (C.c_int, [_MR, C.c_int]),
+    "ctbb_backproject_cpu": (C.c_int, [_MR, C.c_int]),
+    "ctbb_finalize_image_stack_cpu": (C.c_int, [_MR, C.c_float, C.c_int]),
     "ctbb_synth_scan": (C.c_int, [_MR, _P(PhantomParams), C.c_long, C.c_long, _P(C.c_float)]),
     "ctbb_synth_scan_dev": (C.c_int, [_VP, _MR, _P(PhantomParams), C.c_long, C.c_long, _VP]),
 }

@@ -20,7 +20,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libctbb_hip.so")
 EXE = os.path.join(PKG, "ctbangbang")
 
-HOST_SRCS = ["setup.cpp", "filters.cpp", "geometry.cpp", "synth_host.cpp", "dicom.cpp"]
+HOST_SRCS = ["setup.cpp", "filters.cpp", "geometry.cpp", "synth_host.cpp", "dicom.cpp", "cpu_path.cpp"]
 HIP_SRCS = ["bp.hip", "rebin.hip", "filter.hip", "adaptive.hip", "finalize.hip", "synth.hip",
             "ctx.hip", "compat.hip"]
 

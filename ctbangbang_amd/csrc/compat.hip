@@ -13,6 +13,7 @@ namespace {
 
 ctbb_ctx *g_ctx = nullptr;
 std::once_flag g_once;
+ctbb_options g_opt{};      // last options given to ctbb_compat_set_options (the CPU path reads mu_water)
 
 ctbb_ctx *default_ctx() {
     std::call_once(g_once, [] {
@@ -32,10 +33,6 @@ int check(int st, const char *what) {
     std::exit(1);
 }
 
-int no_cpu_path(const char *name) {
-    std::fprintf(stderr, "%s: this build has no CPU reconstruction path (run without --no-gpu)\n", name);
-    std::exit(1);
-}
 
 }  // namespace
 
@@ -57,9 +54,19 @@ int finalize_image_stack(struct recon_metadata *mr) {
     return check(ctbb_finalize_image_stack(default_ctx(), mr), "finalize_image_stack");
 }
 
-int rebin_filter_cpu(struct recon_metadata *) { return no_cpu_path("rebin_filter_cpu"); }
-int backproject_cpu(struct recon_metadata *) { return no_cpu_path("backproject_cpu"); }
-int finalize_image_stack_cpu(struct recon_metadata *) { return no_cpu_path("finalize_image_stack_cpu"); }
+// CPU twins (the reference driver's --no-gpu mode), on every host core
+int rebin_filter_cpu(struct recon_metadata *mr) {
+    int st = ctbb_rebin_filter_cpu(mr, 0);
+    if (st == CTBB_ERR_FILTER) {
+        std::perror("Filter file not found.  You may need to create one.  We hope to add on-the-fly creation in a future update.");
+        std::exit(1);
+    }
+    return check(st, "rebin_filter_cpu");
+}
+int backproject_cpu(struct recon_metadata *mr) { return check(ctbb_backproject_cpu(mr, 0), "backproject_cpu"); }
+int finalize_image_stack_cpu(struct recon_metadata *mr) {
+    return check(ctbb_finalize_image_stack_cpu(mr, g_opt.mu_water, 0), "finalize_image_stack_cpu");
+}
 
 int configure_paths(struct recon_metadata *mr) { return ctbb_configure_paths(mr); }
 
@@ -111,4 +118,8 @@ void parse_config(char *config_file, struct recon_params *structure) {
     }
 }
 
-extern "C" int ctbb_compat_set_options(const struct ctbb_options_s *opt) { return ctbb_ctx_set_options(default_ctx(), opt); }
+extern "C" int ctbb_compat_set_options(const struct ctbb_options_s *opt) {
+    if (!opt) return CTBB_ERR_ARG;
+    g_opt = *opt;
+    return ctbb_ctx_set_options(default_ctx(), opt);
+}

@@ -31,7 +31,8 @@ void usage() {
     std::printf("    Options:\n");
     std::printf("          -v: verbose.\n");
     std::printf("          -t: testing (keep the rebinned sinogram of each block in host memory).\n");
-    std::printf("    --no-gpu: not available in this build (GPU-only).\n");
+    std::printf("    --no-gpu: rebin/filter, backproject and finalize on the host CPU cores\n"
+                "              (the adaptive filter still runs on the GPU, as in the reference).\n");
     std::printf("  --device=i: run on GPU device number 'i'\n");
     std::printf("    --timing: Display timing information for each step of the recon process\n");
     std::printf(" --benchmark: accepted for compatibility (no effect)\n");
@@ -62,10 +63,6 @@ int main(int argc, char **argv) {
         else usage();
     }
     g_verbose = mr.flags.verbose;
-    if (mr.flags.no_gpu) {
-        std::fprintf(stderr, "--no-gpu: this build reconstructs on the GPU only\n");
-        return 1;
-    }
     // device selection: last device unless --device (reference main.cu:125-152)
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
@@ -112,16 +109,19 @@ int main(int argc, char **argv) {
         double t0 = now_ms();
         adaptive_filter_kk(&mr);
         double t1 = now_ms();
-        rebin_filter(&mr);
+        if (mr.flags.no_gpu) rebin_filter_cpu(&mr);
+        else rebin_filter(&mr);
         double t2 = now_ms();
-        backproject(&mr);
+        if (mr.flags.no_gpu) backproject_cpu(&mr);
+        else backproject(&mr);
         double t3 = now_ms();
         t_af += t1 - t0;
         t_rb += t2 - t1;
         t_bp += t3 - t2;
     }
     double t4 = now_ms();
-    finalize_image_stack(&mr);
+    if (mr.flags.no_gpu) finalize_image_stack_cpu(&mr);
+    else finalize_image_stack(&mr);
     double t5 = now_ms();
     logv("Writing image data to %s/%s\n", mr.rp.output_dir, mr.rp.output_file);
     finish_and_cleanup(&mr);

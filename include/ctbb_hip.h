@@ -327,6 +327,30 @@ int ctbb_reconstruct_batch_dev(ctbb_ctx *ctx, int n_scans, struct recon_metadata
                                float *const *d_finals);
 
 /* ------------------------------------------------------------------------ */
+/* CPU reconstruction path (the reference driver's --no-gpu mode).           */
+/* ------------------------------------------------------------------------ */
+
+/* Rebin + ramp filter of mr->ctd.raw into the host array mr->ctd.rebin
+ * ([n_proj_out][n_rows][n_channels_oversampled]) on n_threads host threads
+ * (0: every core): the same coordinate maps and interpolation as the HIP
+ * rebin, an FP64 FFT filter.  Replaces rebin_filter_cpu (reference
+ * include/rebin_filter_cpu.h:27, src/rebin_filter_cpu.cu:75). */
+int ctbb_rebin_filter_cpu(struct recon_metadata *mr, int n_threads);
+
+/* Backprojection of mr->ctd.rebin into block (ri.cb.block_idx - 1) of
+ * mr->ctd.image with the reference CPU twin's conventions (no x flip, qhat
+ * sign +1, layout [z][yi][xi], zero border outside [0, dim - 2]).  Replaces
+ * backproject_cpu (reference include/backproject_cpu.h:26,
+ * src/backproject_cpu.cu:49-154). */
+int ctbb_backproject_cpu(struct recon_metadata *mr, int n_threads);
+
+/* Block reorder + slice thickening (+ HU when mu_water > 0) of mr->ctd.image
+ * into a newly allocated mr->ctd.final_image_stack.  Replaces
+ * finalize_image_stack_cpu (reference include/finalize_image_stack_cpu.h:26,
+ * src/finalize_image_stack_cpu.cu:24-117). */
+int ctbb_finalize_image_stack_cpu(struct recon_metadata *mr, float mu_water, int n_threads);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic helical scans (test data generator).                           */
 /* ------------------------------------------------------------------------ */
 
