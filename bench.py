@@ -4,10 +4,14 @@ blocks -> 40 slices of 3 mm in HU) with the scan resident in HBM.
 
 One step = one complete reconstruction: per block adaptive filter, rebin,
 ramp filter and backprojection, then reorder + thickening + HU.
-Prints one JSON line (see keys below).  --form plain computes in FP32
-throughout (ramp filter as an FP32 FFT convolution by default); --form mixed
-selects the precision form (FP16 hi+lo MFMA Toeplitz filter, BF16 filtered
-projections, FP32 accumulation).
+Prints one JSON line.  `value` is voxel-updates/s with the scan resident in
+HBM (plain FP32 form); `s_per_volume` is the host-to-host time (scan in
+page-locked host memory to the final stack in host memory) and
+`s_per_volume_hbm_resident` the time `value` is computed from.  The same
+line carries the precision form (`forms.mixed`) and the batching form
+(`forms.batched`, several seeded scans of uneven size through one
+pipeline against one call per scan), measured in the same run.
+--form mixed makes the precision form the headline instead.
 """
 import argparse
 import json
@@ -20,7 +24,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BP_BYTES_PER_UPDATE = 16.0   # 4 FP32 bilinear taps per voxel-update
+FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 vector peak (packed; non-packed issue: half)
+BP_BYTES_PER_UPDATE = 16.0   # 4 FP32 bilinear taps per voxel-update (served from LDS)
+BP_FLOP_PER_UPDATE = 12.0    # q 2, row coordinate 2, bilinear 6, accumulate 2
+N_SIMD = 1024                # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9             # peak engine clock
 
 
 def parse():
@@ -35,8 +43,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host-ingest", action="store_true",
                     help="skip the PCIe-inclusive run (scan in page-locked host memory, final stack to host)")
-    ap.add_argument("--cpu-columns", type=int, default=1024, help="voxel columns in the CPU baseline sample")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_traffic_c1.json"))
+    ap.add_argument("--cpu-patch", type=int, default=64,
+                    help="CPU baseline: side of each square sample of voxel columns (one sample per thread)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: the usable cores, at most 16)")
+    ap.add_argument("--no-forms", action="store_true", help="skip the mixed and batched figures of the default line")
+    ap.add_argument("--line-scans", type=int, default=8, help="scans in the batched figure of the default line")
+    ap.add_argument("--counters-file", default=os.path.join(ROOT, "profiles", "bp_kernel_c1.json"))
     ap.add_argument("--stage", default="all", choices=["all", "backproject", "filter", "hu16", "copy"],
                     help="all: whole volume (the benchmark); backproject / filter: only the backprojection / the "
                          "ramp filter (--filter form) of one block (for counter runs); hu16: the int16 output "
@@ -146,10 +158,11 @@ def stage_hu16(args, cb, mr, ctx):
                       "hbm_frac": gbs / 8000.0}), flush=True)
 
 
-def bench_c3(args):
-    """Batching form: --c3-scans synthetic scans of uneven length, pitch, FOV
-    and matrix size, reconstructed (a) one reconstruct_dev call per scan and
-    (b) as one reconstruct_batch_dev pipeline; value = voxel-updates/s of (b)."""
+def batch_run(n_scans, bp_tile):
+    """Batching form on n_scans seeded scans of uneven length, pitch, FOV and
+    matrix size: one reconstruct_dev call per scan (loop) against one
+    reconstruct_batch_dev pipeline (batch), both after a warm-up batch.
+    Returns the timings, voxel-updates and whether both give the same bits."""
     import numpy as np
     import torch
 
@@ -158,9 +171,9 @@ def bench_c3(args):
     d = tempfile.mkdtemp()
     ctx = cb.Context(0)
     ctx.set_options(filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32, mu_water=0.02, adaptive_filter=1,
-                    bp_tile=args.bp_tile)
+                    bp_tile=bp_tile)
     mrs, raws, nreads, finals, vu = [], [], [], [], 0.0
-    for i in range(args.c3_scans):
+    for i in range(n_scans):
         tok = scans.c3_tokens(i)
         prm = os.path.join(d, f"c3_{i}.prm")
         cb.write_prm(prm, tok)
@@ -177,22 +190,37 @@ def bench_c3(args):
         nreads.append(n_read)
         finals.append(torch.empty(cb.block_sizes(mr).final_stack, dtype=torch.float32, device="cuda:0"))
     ctx.synchronize()
-    # warm-up (allocations, tables) with the batch call
-    ctx.reconstruct_batch_dev(mrs, raws, nreads, finals)
-    ctx.synchronize()
+    for mode in (cb.BATCH_PIPELINE, cb.BATCH_PERSISTENT):         # warm-up (allocations, tables)
+        ctx.set_options(batch_mode=mode)
+        ctx.reconstruct_batch_dev(mrs, raws, nreads, finals)
+        ctx.synchronize()
     t0 = time.perf_counter()
     for mr, raw, n, out in zip(mrs, raws, nreads, finals):
         ctx.reconstruct_dev(mr, raw, n, out)
     ctx.synchronize()
     t_loop = time.perf_counter() - t0
-    loop_out = [finals[0].cpu().numpy().copy(), finals[-1].cpu().numpy().copy()]
-    t0 = time.perf_counter()
-    ctx.reconstruct_batch_dev(mrs, raws, nreads, finals)
-    ctx.synchronize()
-    t_batch = time.perf_counter() - t0
-    same = bool(np.array_equal(finals[0].cpu().numpy(), loop_out[0]) and
-                np.array_equal(finals[-1].cpu().numpy(), loop_out[1]))
-    n = args.c3_scans
+    loop_out = [f.cpu().numpy().copy() for f in finals]
+    t_modes, same = {}, True
+    for mode in (cb.BATCH_PIPELINE, cb.BATCH_PERSISTENT):
+        ctx.set_options(batch_mode=mode)
+        t0 = time.perf_counter()
+        ctx.reconstruct_batch_dev(mrs, raws, nreads, finals)
+        ctx.synchronize()
+        t_modes[mode] = time.perf_counter() - t0
+        same = same and all(np.array_equal(f.cpu().numpy(), w) for f, w in zip(finals, loop_out))
+    for mr in mrs:
+        cb.free_metadata(mr)
+    ctx.close()
+    del raws, finals
+    torch.cuda.empty_cache()
+    return {"scans": n_scans, "voxel_updates": vu, "s_batch": t_modes[cb.BATCH_PERSISTENT],
+            "s_pipeline": t_modes[cb.BATCH_PIPELINE], "s_loop": t_loop, "same_bits": bool(same)}
+
+
+def bench_c3(args):
+    """Batching workload alone: value = voxel-updates/s of the batch pipeline."""
+    r = batch_run(args.c3_scans, args.bp_tile)
+    n, vu, t_batch, t_loop = r["scans"], r["voxel_updates"], r["s_batch"], r["s_loop"]
     line = {
         "metric": "voxel-updates/s (batch of scans, one pipeline, scans resident in HBM)",
         "value": vu / t_batch, "unit": "voxel-updates/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -201,46 +229,162 @@ def bench_c3(args):
         "config": {"workload": f"c3: {n} phi+z FFS scans, {vu:.4g} voxel-updates", "parallelism": "1 GPU"},
         "scans_per_s": n / t_batch,
         "loop": {"s": t_loop, "voxel_updates_per_s": vu / t_loop, "scans_per_s": n / t_loop},
+        "pipeline": {"s": r["s_pipeline"], "voxel_updates_per_s": vu / r["s_pipeline"]},
         "batch_over_loop": t_loop / t_batch,
-        "batch_equals_loop": same,
+        "batch_equals_loop": r["same_bits"],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
-    for mr in mrs:
-        cb.free_metadata(mr)
-    ctx.close()
 
 
-def cpu_baseline(mr, raw_block, taps, n_cols):
-    """FP32 CPU restatement (oracle/liboracle32.so, one thread) on one block:
-    rebin + FFT ramp filter of the whole block and the backprojection of
-    n_cols sampled voxel columns (all 32 slices), extrapolated to the block."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
+class _QuietStdout:
+    """Send file descriptor 1 to /dev/null (the reference's backproject_cpu
+    prints a progress line per angle); the JSON line is printed after."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        nul = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(nul, 1)
+        os.close(nul)
+
+    def __exit__(self, *exc):
+        import ctypes
+        ctypes.CDLL(None).fflush(None)      # C stdio buffers go to /dev/null too
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def cpu_baseline(mr, raw_block, taps, patch, threads):
+    """CPU path timed on this host.
+
+    Backprojection: the reference's own backproject_cpu (src/backproject_cpu.cu,
+    compiled from source into oracle/_ref) on samples of patch x patch voxel
+    columns x 32 slices of the block, each a sub-grid of the same voxel size
+    at its own place in the field of view, one sample per thread (threads
+    share nothing: each has its own copy of the metadata and output); once on
+    one thread, once on `threads` threads.  Voxel-updates per second are per
+    update, so they carry over to the whole block.
+    Rebin + ramp filter: the oracle's FP32 restatement (oracle/liboracle32.so
+    and the FP64 numpy FFT) of the whole block, one thread.
+    Falls back to the restatement's backprojection where the reference build
+    is absent (kind "port")."""
+    import ctypes as C
+    import threading
+
     import numpy as np
 
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctbangbang_amd as cb
     import oracle
-    s = cb.block_sizes(mr)
+    info = _cpu_info()
+    if threads <= 0:
+        threads = max(1, min(16, info["usable"]))
     t0 = time.perf_counter()
     reb = oracle.rebin(mr, raw_block, double=False)
     reb = oracle.filter_fft(reb, taps).astype(np.float32)
-    t1 = time.perf_counter()
-    rng = np.random.default_rng(0)
-    idx = rng.choice(mr.rp.nx * mr.rp.ny, size=n_cols, replace=False)
-    cx, cy = (idx // mr.rp.ny).astype(np.int32), (idx % mr.rp.ny).astype(np.int32)
-    oracle.backproject(mr, reb, oracle.BP_GPU, cols=(cx, cy), double=False)
-    t2 = time.perf_counter()
-    t_block = (t1 - t0) + (t2 - t1) * (mr.rp.nx * mr.rp.ny) / n_cols
+    t_prep = time.perf_counter() - t0
+    nx, ny = int(mr.rp.nx), int(mr.rp.ny)
+    dxy = float(mr.rp.recon_fov) / nx
+    updates_per_col = cb.block_sizes(mr).voxel_updates_block / (nx * ny)
+    ref = oracle.ref_lib()
+    # sample centres: a ring of places inside the field of view
+    centres = []
+    for k in range(max(threads, 1)):
+        ang = 2.0 * np.pi * k / max(threads, 1)
+        rad = 0.25 * float(mr.rp.recon_fov) * (0.5 + 0.5 * (k % 2))
+        centres.append((float(mr.rp.x_origin) + rad * np.cos(ang), float(mr.rp.y_origin) + rad * np.sin(ang)))
+
+    def sample_mr(cx, cy):
+        from ctbangbang_amd.structs import ReconMetadata
+        m = ReconMetadata.from_buffer_copy(mr)
+        m.rp.nx = m.rp.ny = patch
+        m.rp.recon_fov = dxy * patch
+        m.rp.x_origin, m.rp.y_origin = cx, cy
+        return m
+
+    if ref is not None:
+        f = getattr(ref, "_Z15backproject_cpuP14recon_metadata")
+        f.argtypes = [C.c_void_p]
+        f.restype = C.c_int
+        kind = "reference"
+
+        def run(k, out):
+            m = sample_mr(*centres[k])
+            nblk = int(m.ri.n_slices_block) * patch * patch
+            img = np.zeros(max(int(m.ri.cb.block_idx), 1) * nblk, dtype=np.float32)
+            m.ctd.rebin = reb.ctypes.data_as(C.POINTER(C.c_float))
+            m.ctd.image = img.ctypes.data_as(C.POINTER(C.c_float))
+            out[k] = f(C.addressof(m))
+    else:
+        kind = "port"
+
+        def run(k, out):
+            m = sample_mr(*centres[k])
+            oracle.backproject(m, reb, oracle.BP_CPU, double=False)
+            out[k] = 0
+
+    res = [None] * threads
+    with _QuietStdout():
+        t0 = time.perf_counter()
+        run(0, res)
+        t_one = time.perf_counter() - t0
+        ths = [threading.Thread(target=run, args=(k, res)) for k in range(threads)]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        t_all = time.perf_counter() - t0
+    assert all(r == 0 for r in res)
+    upd = updates_per_col * patch * patch
+    one_core = upd / t_one
+    all_cores = threads * upd / t_all
     return {
-        "value": s.voxel_updates_block / t_block,
+        "value": all_cores,
         "unit": "voxel-updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"one block of the workload: FP32 rebin + FFT ramp filter of the full block "
-                   f"({t1 - t0:.1f} s) and backprojection of {n_cols} of {mr.rp.nx * mr.rp.ny} voxel columns "
-                   f"x 32 slices ({t2 - t1:.1f} s), extrapolated to the block"),
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"backprojection: {'reference backproject_cpu built from source' if kind == 'reference' else 'oracle FP32 restatement'} "
+                   f"on {threads} samples of {patch}x{patch} voxel columns x 32 slices of one block (same voxel size, "
+                   f"{upd:.3g} voxel-updates each), one per thread, {t_all:.1f} s; 1 thread: {t_one:.1f} s"),
+        "one_core": one_core,
+        "rebin_filter_s_per_block": t_prep,
+        "rebin_filter_note": "oracle FP32 rebin + FP64 FFT filter of the whole block, one thread "
+                             "(the reference's rebin_filter_cpu needs FFTW3, absent here)",
+        "host_to_host_block_s_all_cores": t_prep + cb.block_sizes(mr).voxel_updates_block / all_cores,
+        "nproc": info["nproc"], "usable_cpus": info["usable"], "cpu_model": info["model"],
     }
+
+
+def run_volume(ctx, mr, raw, n_read, final, image, steps, warmup):
+    """Time `steps` whole reconstructions (scan resident in HBM) after
+    `warmup`; returns seconds per volume."""
+    import torch
+    for _ in range(warmup):
+        ctx.reconstruct_dev(mr, raw, n_read, final, image)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.reconstruct_dev(mr, raw, n_read, final, image)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
 
 
 def main():
@@ -268,12 +412,12 @@ def main():
     ctx = cb.Context(0)
     mixed = args.form == "mixed"
     modes = {"fft32": cb.FILTER_FFT32, "f32": cb.FILTER_F32, "bf16x3": cb.FILTER_BF16X3, "f16x3": cb.FILTER_F16X3}
-    filt = args.filter or ("f16x3" if mixed else "fft32")
-    ctx.set_options(filter_mode=modes[filt],
-                    proj_dtype=cb.PROJ_BF16 if mixed else cb.PROJ_F32,
-                    mu_water=float(tok.get("MuWater", 0.0)),
-                    adaptive_filter=1 if float(tok["AdaptiveFiltration"]) > 0 else 0,
-                    bp_tile=args.bp_tile)
+    forms = {"plain": {"filter": "fft32", "proj": cb.PROJ_F32},
+             "mixed": {"filter": "f16x3", "proj": cb.PROJ_BF16}}
+    filt = args.filter or forms[args.form]["filter"]
+    base_opts = dict(mu_water=float(tok.get("MuWater", 0.0)),
+                     adaptive_filter=1 if float(tok["AdaptiveFiltration"]) > 0 else 0, bp_tile=args.bp_tile)
+    ctx.set_options(filter_mode=modes[filt], proj_dtype=forms[args.form]["proj"], **base_opts)
     if args.stage == "hu16":
         stage_hu16(args, cb, mr, ctx)
         ctx.close()
@@ -299,32 +443,41 @@ def main():
         vu_blocks.append(cb.block_sizes(mr).voxel_updates_block)
     vu_total = float(sum(vu_blocks))
 
-    for _ in range(args.warmup):
-        ctx.reconstruct_dev(mr, raw, n_read, final, image)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.reconstruct_dev(mr, raw, n_read, final, image)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    dt = run_volume(ctx, mr, raw, n_read, final, image, args.steps, args.warmup)
 
-    # per-kernel device time of the last step (HIP events on the ctx stream)
+    # per-kernel device time of the last step (HIP events on the streams the
+    # kernels run on)
     fam = {}
     for f in ("adaptive", "rebin", "filter", "backproject", "finalize"):
         ms, n = ctx.kernel_time(f)
         fam[f] = {"ms_total": round(ms, 3), "launches": n}
     bp_ms, bp_n = ctx.kernel_time("backproject")
     bp_avg_s = bp_ms / max(bp_n, 1) / 1e3
-    vu_launch = vu_blocks[-1]
-    achieved = BP_BYTES_PER_UPDATE * vu_launch / bp_avg_s / 1e9
-    traffic = None
-    if os.path.exists(args.traffic_file):
+    vu_launch = vu_total / len(vu_blocks)          # mean updates per backprojection launch
+    counters = {}
+    if os.path.exists(args.counters_file):
         try:
-            traffic = json.load(open(args.traffic_file)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+            counters = json.load(open(args.counters_file))
+        except (OSError, ValueError):
+            counters = {}
+    traffic = counters.get("bytes_per_launch")
+    achieved = BP_FLOP_PER_UPDATE * vu_launch / bp_avg_s / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+            "kernel": "bp_kernel (backprojection)",
+            "per_unit": f"{BP_FLOP_PER_UPDATE:.0f} FP32 FLOP per voxel-update (q 2, row coordinate 2, bilinear 6, "
+                        f"accumulate 2) x {vu_launch:.4g} updates per launch",
+            "avg_launch_ms": bp_avg_s * 1e3}
+    if counters.get("valu_insts_per_launch"):
+        # issue slots: a wave64 VALU instruction occupies its SIMD for 4 cycles
+        roof["valu_issue_frac"] = 4.0 * counters["valu_insts_per_launch"] / (N_SIMD * CLOCK_HZ * bp_avg_s)
+        roof["valu_issue_note"] = ("SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz x launch time): the roof that "
+                                   "binds; counters from " + os.path.relpath(args.counters_file, ROOT))
+    if traffic:
+        roof["hbm_GB_per_s"] = traffic / bp_avg_s / 1e9
+        roof["hbm_frac"] = roof["hbm_GB_per_s"] / HBM_PEAK_GBS
+    roof["lds_gather_GB_per_s"] = BP_BYTES_PER_UPDATE * vu_launch / bp_avg_s / 1e9
+    roof["lds_gather_note"] = "16 B of bilinear taps per voxel-update, read from LDS (not HBM traffic)"
 
     line = {
         "metric": "voxel-updates/s (end-to-end, whole volume, scan resident in HBM)",
@@ -334,7 +487,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": dt * 1e3,
-        "s_per_volume": dt,
+        "s_per_volume": None,
+        "s_per_volume_hbm_resident": dt,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -342,19 +496,16 @@ def main():
         "data": "synthetic (analytic Shepp-Logan-type phantom + 8 seeded ellipsoids, Poisson noise I0=1e5)",
         "config": {"workload": f"{args.config}: {int(mr.rp.nx)}^2 x {int(mr.ri.n_slices_recon)} raw slices, "
                                f"{int(mr.ri.n_blocks)} blocks, n_ffs={int(mr.ri.n_ffs)}, "
-                               f"{int(mr.cg.n_rows)} rows, {n_read} readings",
+                               f"{int(mr.cg.n_rows)} rows, {n_read} readings, adaptive filter "
+                               f"{'KK' if float(tok['AdaptiveFiltration']) > 0 else 'off'}",
                    "form": args.form, "filter": filt, "voxel_updates": vu_total, "parallelism": "1 GPU"},
         "backproject_updates_per_s": vu_launch / bp_avg_s,
         "kernels_last_step": fam,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "bp_kernel (backprojection)",
-                     "per_unit": "16 B per voxel-update (4 FP32 bilinear taps)",
-                     "avg_launch_ms": bp_avg_s * 1e3},
+        "roofline": roof,
     }
     if not args.no_host_ingest:
-        # same reconstruction from a page-locked host copy of the scan to a
-        # host final stack: uploads overlap the compute (reconstruct_host)
+        # s/volume: the same reconstruction from a page-locked host copy of the
+        # scan to a host final stack (uploads overlap the compute)
         h_raw = torch.empty(n_read * frame, dtype=torch.float32, pin_memory=True)
         h_raw.copy_(raw)
         h_fin = torch.empty(s.final_stack, dtype=torch.float32, pin_memory=True)
@@ -364,19 +515,45 @@ def main():
         for _ in range(n_h):
             ctx.reconstruct_host(mr, h_raw, n_read, h_fin)
         dt_h = (time.perf_counter() - t0) / n_h
-        line["host_ingest"] = {"s_per_volume": dt_h, "voxel_updates_per_s": vu_total / dt_h, "runs": n_h,
-                               "note": "scan in page-locked host memory to final stack in host memory "
-                                       "(H2D of each pull overlapped with the previous block)"}
+        line["s_per_volume"] = dt_h
+        line["host_to_host"] = {"s_per_volume": dt_h, "voxel_updates_per_s": vu_total / dt_h, "runs": n_h,
+                                "note": "scan in page-locked host memory to final stack in host memory "
+                                        "(H2D of each pull overlapped with the previous block)"}
         del h_raw, h_fin
+    if not args.no_forms and args.config == "c1":
+        other = "plain" if mixed else "mixed"
+        ctx.set_options(filter_mode=modes[forms[other]["filter"]], proj_dtype=forms[other]["proj"], **base_opts)
+        dt_o = run_volume(ctx, mr, raw, n_read, final, image, max(1, min(args.steps, 5)), 1)
+        ctx.set_options(filter_mode=modes[filt], proj_dtype=forms[args.form]["proj"], **base_opts)
+        f_this = {"s_per_volume_hbm_resident": dt, "voxel_updates_per_s": vu_total / dt,
+                  "filter": filt, "projections": "bf16" if mixed else "f32"}
+        f_other = {"s_per_volume_hbm_resident": dt_o, "voxel_updates_per_s": vu_total / dt_o,
+                   "filter": forms[other]["filter"], "projections": "bf16" if other == "mixed" else "f32"}
+        plain_f, mixed_f = (f_other, f_this) if mixed else (f_this, f_other)
+        line["forms"] = {"plain": plain_f, "mixed": mixed_f,
+                         "mixed_over_plain": plain_f["s_per_volume_hbm_resident"] / mixed_f["s_per_volume_hbm_resident"]}
     if not args.no_cpu_baseline:
-        cb.plan_block(mr, 0)
+        cb.plan_block(mr, int(mr.ri.n_blocks) // 2)
         s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
         rb = raw[s0 * frame:(s0 + n) * frame].cpu().numpy()
-        line["cpu_baseline"] = cpu_baseline(mr, rb, cb.load_filter(mr), args.cpu_columns)
+        line["cpu_baseline"] = cpu_baseline(mr, rb, cb.load_filter(mr), args.cpu_patch, args.cpu_threads)
     else:
         line["cpu_baseline"] = None
-    print(json.dumps(line), flush=True)
     ctx.close()
+    del raw, final, image
+    torch.cuda.empty_cache()
+    if not args.no_forms and args.config == "c1" and args.line_scans > 0:
+        r = batch_run(args.line_scans, args.bp_tile)
+        line.setdefault("forms", {})["batched"] = {
+            "scans": r["scans"], "voxel_updates": r["voxel_updates"],
+            "batch_s": r["s_batch"], "pipeline_s": r["s_pipeline"], "loop_s": r["s_loop"],
+            "batch_voxel_updates_per_s": r["voxel_updates"] / r["s_batch"],
+            "loop_voxel_updates_per_s": r["voxel_updates"] / r["s_loop"],
+            "batch_over_loop": r["s_loop"] / r["s_batch"], "same_bits": r["same_bits"],
+            "workload": "the first scans of the c3 batch (seeds 3000+i): uneven rotations, pitch, FOV, matrix",
+            "form": "batch: scans grouped per kernel variant, captured preparation graph + one persistent "
+                    "backprojection launch per group; pipeline: one launch per block; loop: one call per scan"}
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

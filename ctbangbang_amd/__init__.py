@@ -32,6 +32,7 @@ __all__ = [
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 FILTER_F32, FILTER_BF16X3, FILTER_F16X3, FILTER_FFT32 = 0, 1, 2, 3
 PROJ_F32, PROJ_BF16 = 0, 1
+BATCH_PERSISTENT, BATCH_PIPELINE = 0, 1
 NOALLOC_MAX_VIEWS = 16384      # CTBB_NOALLOC_MAX_VIEWS of include/ctbb_hip.h
 
 _STATUS = {
@@ -109,6 +110,7 @@ _SIGS = {
     "ctbb_rebin_filter_cpu": (C.c_int, [_MR, C.c_int]),
     "ctbb_backproject_cpu": (C.c_int, [_MR, C.c_int]),
     "ctbb_finalize_image_stack_cpu": (C.c_int, [_MR, C.c_float, C.c_int]),
+    "ctbb_plan_batch_dev": (C.c_int, [_VP, C.c_int, C.POINTER(_MR), _VP, C.c_int]),
     "ctbb_synth_scan": (C.c_int, [_MR, _P(PhantomParams), C.c_long, C.c_long, _P(C.c_float)]),
     "ctbb_synth_scan_dev": (C.c_int, [_VP, _MR, _P(PhantomParams), C.c_long, C.c_long, _VP]),
 }
@@ -358,6 +360,18 @@ class Context:
         n_arr = (C.c_size_t * n)(*[int(x) for x in n_raw_readings])
         f_arr = (_VP * n)(*[_ptr(d) for d in d_finals])
         _check(lib().ctbb_reconstruct_batch_dev(self._h, n, m_arr, r_arr, n_arr, f_arr), "reconstruct_batch_dev")
+
+    def plan_batch_dev(self, mrs):
+        """Block plans of a batch computed on the device (the batching form's
+        work list): a structured array with one row per (scan, block)."""
+        n = sum(int(m.ri.n_blocks) for m in mrs)
+        dt = np.dtype([("scan", "<i4"), ("block", "<i4"), ("idx_pull_start", "<i4"), ("n_proj_pull", "<i4"),
+                       ("n_half_turns", "<i4"), ("idx_block_slice_start", "<i4"), ("idx_block_slice_end", "<i4"),
+                       ("block_slice_start", "<f4"), ("block_slice_end", "<f4"), ("tube_start", "<f4")])
+        out = np.zeros(n, dtype=dt)
+        m_arr = (_MR * len(mrs))(*[C.pointer(m) for m in mrs])
+        _check(lib().ctbb_plan_batch_dev(self._h, len(mrs), m_arr, out.ctypes.data, n), "plan_batch_dev")
+        return out
 
     def synth_scan_dev(self, mr, pp, first, count, d_out):
         _check(lib().ctbb_synth_scan_dev(self._h, C.byref(mr), C.byref(pp), int(first), int(count), _ptr(d_out)),

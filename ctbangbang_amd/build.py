@@ -21,7 +21,7 @@ LIB = os.path.join(PKG, "libctbb_hip.so")
 EXE = os.path.join(PKG, "ctbangbang")
 
 HOST_SRCS = ["setup.cpp", "filters.cpp", "geometry.cpp", "synth_host.cpp", "dicom.cpp", "cpu_path.cpp"]
-HIP_SRCS = ["bp.hip", "rebin.hip", "filter.hip", "adaptive.hip", "finalize.hip", "synth.hip",
+HIP_SRCS = ["bp.hip", "rebin.hip", "filter.hip", "adaptive.hip", "finalize.hip", "synth.hip", "plan.hip",
             "ctx.hip", "compat.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -34,6 +34,9 @@
 // contiguous: their footprints on each view overlap and hit in that XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace ctbb {
@@ -66,8 +69,8 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
 // Raw bits of one filtered-projection sample (float32 bits, or bfloat16 bits
 // in the low half).  Conversion happens when the prefetched window is written
 // to LDS, so the loads stay in flight while the previous view is computed.
-template <bool kBf16>
-__device__ __forceinline__ uint32_t load_bits(const BpArgs &a, size_t i) {
+template <bool kBf16, class Args>
+__device__ __forceinline__ uint32_t load_bits(Args &a, size_t i) {
     if (kBf16) return a.rebin_bf16[i];
     return __float_as_uint(a.rebin[i]);
 }
@@ -112,24 +115,21 @@ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 // ZS z-groups of NZ slices share one workgroup (256 * ZS threads): thread t
 // takes column t % 256 and slices (t / 256) * NZ ... + NZ - 1 of the block.
 // NZ = 32, ZS = 1 keeps 96 accumulators per thread (150 VGPRs, 3 waves per
-// SIMD); NZ = 16, ZS = 2 halves them (about 95 VGPRs, 4 waves per SIMD) and
+// SIMD); NZ = 16, ZS = 2 halves them (120 VGPRs, 4 waves per SIMD) and
 // shares one staged window between the two halves.
-template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
-__global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 1 ? 3 : 4))) void bp_kernel(
-    BpArgs a) {
+// One 16x16-column tile (all 32 slices) of the block described by `a`, by
+// the whole workgroup; zblk selects the slice range when a launch splits
+// the 32 slices over several workgroups (NZ * ZS < 32).  lds2: the dynamic
+// LDS (2 window buffers x [rows_pad + 1][RS] pairs, a ring of 128
+// view-table entries, with two z-groups 2 x 256 column geometries, and the
+// ring's 128 window bases).
+// The workgroup leaves LDS free for reuse once every thread has returned.
+// Args: BpArgs, either a kernel argument or a job read through the scalar
+// cache (constant address space), so its fields stay in scalar registers.
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16, class Args>
+__device__ __forceinline__ void bp_tile(Args &a, int tile, int zblk, float2 *lds2) {
     constexpr int THREADS = 256 * ZS;
-    // dynamic LDS: 2 window buffers x [rows_pad + 1][RS] pairs, a ring of
-    // 128 view-table entries, and (two z-groups) 2 x 256 column geometries
-    extern __shared__ float2 lds2[];
     const int tid = threadIdx.x;
-
-    // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give each
-    // XCD a contiguous range of tiles.
-    int tile = blockIdx.x;
-    if ((a.n_tiles & 7) == 0) {
-        const int per = a.n_tiles >> 3;
-        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    }
     const int tx = tile / a.tiles_y, ty = tile - tx * a.tiles_y;
     // Column of this thread in the 16x16 tile.  Each wave covers 4 x-rows of
     // 16; within it each group of 16 lanes (one LDS cycle of a 64-bit read)
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     // slice z0 + ZSTEP * zc + c.
     static_assert(ZS == 1 || NZ * ZS == 32, "two z-groups split the block's 32 slices");
     constexpr int ZSTEP = ZS == 2 ? 2 : 1;
-    const int z0 = ZS == 2 ? blockIdx.y * 32 + 4 * (tid >> 8) : blockIdx.y * NZ;
+    const int z0 = ZS == 2 ? zblk * 32 + 4 * (tid >> 8) : zblk * NZ;
 
     const float x = a.xs * (static_cast<float>(xi) - a.xc) + a.x0;
     const float y = a.ys * (static_cast<float>(yi) - a.yc) + a.y0;
@@ -241,9 +241,14 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         }
     };
 
-    float s[NZ], st[NZ], ht[NZ];
+    // s: the image; acc = (s_t, h_t + 1e-30) of the current angle.  A
+    // nonzero weight is at least 7.8e-10, so starting h_t at 1e-30 instead
+    // of adding it before the reciprocal gives the same bits, and an angle
+    // no view sees ends with s_t = 0, h_t = 1e-30 and adds 0.
+    float s[NZ];
+    float2 acc[NZ];
 #pragma unroll
-    for (int z = 0; z < NZ; z++) { s[z] = 0.0f; st[z] = 0.0f; ht[z] = 0.0f; }
+    for (int z = 0; z < NZ; z++) { s[z] = 0.0f; acc[z] = make_float2(0.0f, 1e-30f); }
 
     // Views run in reference order t = i * n_ht + k (angle i, half turn k);
     // the table holds them in that order.  Pipeline: while view t is
@@ -254,8 +259,16 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     const int n_views = a.half * a.n_ht;
     const float4 *vtab = reinterpret_cast<const float4 *>(a.vtab);
     float4 *ring = reinterpret_cast<float4 *>(lds2 + 2 * n_stage);
+    // Each entry's window base (the staged window's first channel for this
+    // tile, the same in every lane) is computed once, by the thread that
+    // loads the entry, into a parallel ring of ints.
+    int *plo_ring = reinterpret_cast<int *>(ring + 128 + (ZS == 2 ? 512 : 0));
     auto load_chunk = [&](int c) {
-        if (tid < 64) ring[(c & 1) * 64 + tid] = vtab[min(64 * c + tid, n_views - 1)];
+        if (tid < 64) {
+            const float4 e = vtab[min(64 * c + tid, n_views - 1)];
+            ring[(c & 1) * 64 + tid] = e;
+            plo_ring[(c & 1) * 64 + tid] = window_lo(e);
+        }
     };
 
     // Per-column geometry of a view (window base plo), shared by the slices:
@@ -291,7 +304,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
     int plo_next;
     {
         const float4 v0 = ring[0];
-        plo_next = window_lo(v0);
+        plo_next = plo_ring[0];
         fetch(__float_as_int(v0.w), plo_next);
         if (ZS == 2 && zg == 0) {
             float fp, g, dz;
@@ -333,7 +346,7 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
         const float4 vw = ring[t & 127];
         if (t + 1 < n_views) {
             const float4 vn = ring[(t + 1) & 127];
-            plo_next = window_lo(vn);
+            plo_next = plo_ring[(t + 1) & 127];
             __builtin_amdgcn_s_setprio(1);
             fetch(__float_as_int(vn.w), plo_next);
             __builtin_amdgcn_s_setprio(0);
@@ -419,29 +432,123 @@ __global__ __launch_bounds__(256 * ZS) __attribute__((amdgpu_waves_per_eu(ZS == 
                 const float top = fmaf(fp, e0[c].y, e0[c].x);
                 const float bot = fmaf(fp, e1[c].y, e1[c].x);
                 const float val = fmaf(fr, bot - top, top);
-                st[zc + c] = fmaf(val, w[c], st[zc + c]);
-                ht[zc + c] += w[c];
+                acc[zc + c].x = fmaf(val, w[c], acc[zc + c].x);
+                acc[zc + c].y += w[c];
             }
         }
         if (last_k) {
-            // s += (1/h_t) s_t where h_t != 0 (reference include/backproject.cuh:104-108).
-            // A nonzero weight is at least 7.8e-10, so adding 1e-30 leaves
-            // a nonzero h_t unchanged, and h_t = 0 (then s_t = 0) becomes
-            // 0 * 1e30 = 0 without a select.
+            // s += (1/h_t) s_t where h_t != 0 (reference include/backproject.cuh:104-108);
+            // h_t = 0 (then s_t = 0) is 0 * 1e30 = 0 without a select.
 #pragma unroll
             for (int z = 0; z < NZ; z++) {
-                s[z] = fmaf(st[z], __builtin_amdgcn_rcpf(ht[z] + 1e-30f), s[z]);
-                st[z] = 0.0f;
-                ht[z] = 0.0f;
+                s[z] = fmaf(acc[z].x, __builtin_amdgcn_rcpf(acc[z].y), s[z]);
+                acc[z] = make_float2(0.0f, 1e-30f);
             }
         }
     }
 
     const bool in_fov = sqrtf(x * x + y * y) < a.acq_half;
+    // slice bases in scalar registers (z0 is the same across a wave), the
+    // column's 32-bit offset per lane
     const size_t nxy = static_cast<size_t>(a.nx) * a.ny;
-    float *o = a.out + static_cast<size_t>(xi) * a.ny + yi + static_cast<size_t>(z0) * nxy;
+    const int zw = __builtin_amdgcn_readfirstlane(z0);
+    const unsigned lane_off = static_cast<unsigned>(xi * a.ny + yi);
 #pragma unroll
-    for (int z = 0; z < NZ; z++) o[(ZSTEP * (z & ~3) + (z & 3)) * nxy] = in_fov ? s[z] * a.out_scale : 0.0f;
+    for (int z = 0; z < NZ; z++) {
+        float *os = a.out + static_cast<size_t>(zw + ZSTEP * (z & ~3) + (z & 3)) * nxy;
+        os[lane_off] = in_fov ? s[z] * a.out_scale : 0.0f;
+    }
+}
+
+// One launch per block: workgroup = tile (x) and slice range (y).
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__device__ __forceinline__ void bp_block_body(const BpArgs &a, float2 *lds2) {
+    // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give each
+    // XCD a contiguous range of tiles.
+    int tile = blockIdx.x;
+    if ((a.n_tiles & 7) == 0) {
+        const int per = a.n_tiles >> 3;
+        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    }
+    bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, blockIdx.y, lds2);
+}
+
+// Two z-groups (512 threads, 4 waves per SIMD; at most 128 VGPRs, 120 for
+// the usual shapes, which leaves room for the next block's low-register
+// preparation kernels beside it).
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void bp_kernel2(BpArgs a) {
+    extern __shared__ float2 lds2[];
+    bp_block_body<NZ, CPT, TPR, ZS, kBf16>(a, lds2);
+}
+
+// One z-group of 32 slices (256 threads, 3 waves per SIMD).
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bp_kernel1(BpArgs a) {
+    extern __shared__ float2 lds2[];
+    bp_block_body<NZ, CPT, TPR, ZS, kBf16>(a, lds2);
+}
+
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+constexpr auto bp_kernel_of() {
+    if constexpr (ZS == 1) return &bp_kernel1<NZ, CPT, TPR, ZS, kBf16>;
+    else return &bp_kernel2<NZ, CPT, TPR, ZS, kBf16>;
+}
+
+// Persistent form: a grid of at most the resident workgroups works through
+// the (block, tile) items of many blocks (jobs).  Eight queues, one per XCD
+// (workgroup b starts on queue b % 8, the XCD the hardware dispatches it to):
+// queue q holds band q of every job, the contiguous tiles
+// [floor(q n / 8), floor((q + 1) n / 8)) of its n tiles, in job order, so
+// the tiles an XCD runs at one time lie side by side and share its L2.  A
+// workgroup takes items from its own queue, then from the others in turn,
+// and returns when all eight are empty (every workgroup reaches that).
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__device__ __forceinline__ void bp_persistent_body(const BpQueues &Q, float2 *lds2) {
+    __shared__ int item_slot;
+    const int q0 = blockIdx.x & 7;
+    for (int qq = 0; qq < 8; qq++) {
+        const int q = (q0 + qq) & 7;
+        const int *pre = Q.band_pre + q * (Q.n_jobs + 1);
+        const int len = __builtin_amdgcn_readfirstlane(pre[Q.n_jobs]);
+        for (;;) {
+            if (threadIdx.x == 0) item_slot = atomicAdd(Q.counters + q, 1);
+            __syncthreads();
+            // (the same value in every lane: scalar from here on)
+            const int item = __builtin_amdgcn_readfirstlane(item_slot);
+            __syncthreads();                    // every thread has read the slot (and left the last tile)
+            if (item >= len) break;
+            // the job holding this item: the last j with pre[j] <= item
+            int lo = 0, hi = Q.n_jobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (__builtin_amdgcn_readfirstlane(pre[mid]) <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            // the job through the scalar cache: the plan kernels wrote the
+            // jobs before this launch and nothing writes them during it
+            const __attribute__((address_space(4))) BpArgs &a =
+                *(const __attribute__((address_space(4))) BpArgs *)(Q.jobs + lo);
+            const int tile = (q * a.n_tiles) / 8 + (item - __builtin_amdgcn_readfirstlane(pre[lo]));
+            bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, 0, lds2);
+        }
+    }
+}
+
+// Two z-groups (512 threads): at most 120 VGPRs (the attribute counts VGPR
+// pairs on gfx950), so the next group's low-register preparation kernels
+// fit beside it.
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) __attribute__((amdgpu_num_vgpr(60)))
+void bp_persistent2_kernel(BpQueues Q) {
+    extern __shared__ float2 lds2[];
+    bp_persistent_body<NZ, CPT, TPR, ZS, kBf16>(Q, lds2);
+}
+
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bp_persistent1_kernel(BpQueues Q) {
+    extern __shared__ float2 lds2[];
+    bp_persistent_body<NZ, CPT, TPR, ZS, kBf16>(Q, lds2);
 }
 
 // View table of one block, built on the device (the backprojection's work
@@ -455,116 +562,211 @@ __global__ __launch_bounds__(256) void view_table_kernel(BpView *tab, double tub
                                                           int n_proj_turn, double z_rot) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= half * n_ht) return;
-    const int i = e / n_ht, k = e - i * n_ht;
-    const double pi = static_cast<double>(kPiBp);
-    const double dtheta = __dadd_rn(__dmul_rn(__ddiv_rn(2.0 * pi, static_cast<double>(n_proj_turn)), i),
-                                    __dmul_rn(static_cast<double>(k), pi));
-    const double th = __dadd_rn(tube_start, dtheta);
-    BpView v;
-    v.sin_t = static_cast<float>(sin(th));
-    v.cos_t = static_cast<float>(cos(th));
-    v.c_v = static_cast<float>(__dsub_rn(z_rot / 2.0, __ddiv_rn(__dmul_rn(z_rot, dtheta), 2.0 * pi)));
-    v.proj = i + k * half;
-    tab[e] = v;
+    view_table_entry(tab + e, e, tube_start, half, n_ht, n_proj_turn, z_rot);
+}
+
+template <int NZ, int CPT, int TPR, int ZS>
+size_t lds_bytes_of(const BpArgs &a) {
+    const int rows_pad = (256 * ZS) >> a.tpr_log2;
+    return 2 * static_cast<size_t>(rows_pad + 1) * a.tpr * (CPT + kPad<CPT>) * sizeof(float2) +
+           128 * sizeof(float4) + (ZS == 2 ? 2 * 256 * sizeof(float4) : 0) + 128 * sizeof(int);
+}
+
+// Windows above 64 KB (tall detectors): raise the kernel's LDS limit, up to
+// the device's per-workgroup maximum.
+int allow_lds(const void *kf, size_t lds_bytes) {
+    if (lds_bytes <= 65536) return CTBB_OK;
+    int dev = 0, max_lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return CTBB_ERR_HIP;
+    if (lds_bytes > static_cast<size_t>(max_lds)) return CTBB_ERR_SHAPE;
+    if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes)) != hipSuccess)
+        return CTBB_ERR_SHAPE;
+    return CTBB_OK;
 }
 
 template <int NZ, int CPT, int TPR, int ZS>
 int launch_c(const BpArgs &a, hipStream_t stream) {
-    const int rows_pad = (256 * ZS) >> a.tpr_log2;
-    const size_t lds_bytes = 2 * static_cast<size_t>(rows_pad + 1) * a.tpr * (CPT + kPad<CPT>) * sizeof(float2) +
-                             128 * sizeof(float4) + (ZS == 2 ? 2 * 256 * sizeof(float4) : 0);
-    auto kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, true>)
-                           : reinterpret_cast<const void *>(&bp_kernel<NZ, CPT, TPR, ZS, false>);
-    if (lds_bytes > 65536) {
-        // tall detectors: up to the device's per-workgroup LDS
-        int dev = 0, max_lds = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            return CTBB_ERR_HIP;
-        if (lds_bytes > static_cast<size_t>(max_lds)) return CTBB_ERR_SHAPE;
-        if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes)) !=
-            hipSuccess)
-            return CTBB_ERR_SHAPE;
-    }
+    const size_t lds_bytes = lds_bytes_of<NZ, CPT, TPR, ZS>(a);
+    auto kf = a.rebin_bf16 ? reinterpret_cast<const void *>(bp_kernel_of<NZ, CPT, TPR, ZS, true>())
+                           : reinterpret_cast<const void *>(bp_kernel_of<NZ, CPT, TPR, ZS, false>());
+    const int st = allow_lds(kf, lds_bytes);
+    if (st) return st;
     dim3 grid(a.n_tiles, 32 / (NZ * ZS));
-    if (a.rebin_bf16)
-        hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, ZS, true>), grid, dim3(256 * ZS), lds_bytes, stream, a);
+    void *args[] = {const_cast<BpArgs *>(&a)};
+    return hipLaunchKernel(kf, grid, dim3(256 * ZS), args, lds_bytes, stream) == hipSuccess ? CTBB_OK
+                                                                                           : CTBB_ERR_HIP;
+}
+
+template <int NZ, int CPT, int TPR, int ZS>
+int launch_p(const BpArgs &a, const BpQueues &Q, int total_items, int max_groups, hipStream_t stream) {
+    const size_t lds_bytes = lds_bytes_of<NZ, CPT, TPR, ZS>(a);
+    const void *kf;
+    if constexpr (ZS == 2)
+        kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_persistent2_kernel<NZ, CPT, TPR, ZS, true>)
+                          : reinterpret_cast<const void *>(&bp_persistent2_kernel<NZ, CPT, TPR, ZS, false>);
     else
-        hipLaunchKernelGGL((bp_kernel<NZ, CPT, TPR, ZS, false>), grid, dim3(256 * ZS), lds_bytes, stream, a);
+        kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_persistent1_kernel<NZ, CPT, TPR, ZS, true>)
+                          : reinterpret_cast<const void *>(&bp_persistent1_kernel<NZ, CPT, TPR, ZS, false>);
+    int st = allow_lds(kf, lds_bytes);
+    if (st) return st;
+    int dev = 0, n_cu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 256 * ZS, lds_bytes) != hipSuccess)
+        return CTBB_ERR_HIP;
+    long groups = static_cast<long>(std::max(per_cu, 1)) * std::max(n_cu, 1);
+    if (max_groups > 0) groups = std::min<long>(groups, max_groups);
+    groups = std::min<long>(groups, std::max(total_items, 1));
+    groups = std::max<long>(8, (groups + 7) / 8 * 8);     // every queue has workgroups starting on it
+    if constexpr (ZS == 2) {
+        if (a.rebin_bf16)
+            hipLaunchKernelGGL((bp_persistent2_kernel<NZ, CPT, TPR, ZS, true>), dim3(static_cast<unsigned>(groups)),
+                               dim3(256 * ZS), lds_bytes, stream, Q);
+        else
+            hipLaunchKernelGGL((bp_persistent2_kernel<NZ, CPT, TPR, ZS, false>), dim3(static_cast<unsigned>(groups)),
+                               dim3(256 * ZS), lds_bytes, stream, Q);
+    } else {
+        if (a.rebin_bf16)
+            hipLaunchKernelGGL((bp_persistent1_kernel<NZ, CPT, TPR, ZS, true>), dim3(static_cast<unsigned>(groups)),
+                               dim3(256 * ZS), lds_bytes, stream, Q);
+        else
+            hipLaunchKernelGGL((bp_persistent1_kernel<NZ, CPT, TPR, ZS, false>), dim3(static_cast<unsigned>(groups)),
+                               dim3(256 * ZS), lds_bytes, stream, Q);
+    }
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
 
-// Channels per thread: the smallest supported CPT whose window
-// W = tpr * CPT covers the tile footprint (2 half_w + 3 channels).  The
-// (TPR, CPT) shapes of 16-, 32- and 64-row detectors at the usual fields of
-// view have kernels with compile-time strides; others use the generic one.
-template <int NZ, int ZS>
-int launch_nz(BpArgs a, hipStream_t stream) {
-    constexpr int kThreads = 256 * ZS;
-    const float need = 2.0f * a.half_w + 3.0f;
-    int rows_pad = 2;
-    while (rows_pad < a.n_rows) rows_pad <<= 1;
-    if (rows_pad > kThreads) return CTBB_ERR_SHAPE;
-    a.tpr = kThreads / rows_pad;
-    a.tpr_log2 = 0;
-    while ((1 << a.tpr_log2) < a.tpr) a.tpr_log2++;
-    const int cpt = static_cast<int>(ceilf(need / static_cast<float>(a.tpr)));
-    static const int kCpt[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 14};
-    for (int c : kCpt) {
-        if (c < cpt) continue;
-        const int key = a.tpr * 100 + c;
-        if constexpr (ZS == 1) {
-            switch (key) {
-                case 406: return launch_c<NZ, 6, 4, ZS>(a, stream);
-                case 408: return launch_c<NZ, 8, 4, ZS>(a, stream);
-                case 410: return launch_c<NZ, 10, 4, ZS>(a, stream);
-                case 412: return launch_c<NZ, 12, 4, ZS>(a, stream);
-                case 414: return launch_c<NZ, 14, 4, ZS>(a, stream);
-                case 804: return launch_c<NZ, 4, 8, ZS>(a, stream);
-                case 806: return launch_c<NZ, 6, 8, ZS>(a, stream);
-                case 808: return launch_c<NZ, 8, 8, ZS>(a, stream);
-                case 1602: return launch_c<NZ, 2, 16, ZS>(a, stream);
-                case 1604: return launch_c<NZ, 4, 16, ZS>(a, stream);
-                default: break;
-            }
-        } else {
-            switch (key) {
-                case 804: return launch_c<NZ, 4, 8, ZS>(a, stream);
-                case 805: return launch_c<NZ, 5, 8, ZS>(a, stream);
-                case 806: return launch_c<NZ, 6, 8, ZS>(a, stream);
-                case 1602: return launch_c<NZ, 2, 16, ZS>(a, stream);
-                case 1603: return launch_c<NZ, 3, 16, ZS>(a, stream);
-                case 3201: return launch_c<NZ, 1, 32, ZS>(a, stream);
-                case 3202: return launch_c<NZ, 2, 32, ZS>(a, stream);
-                default: break;
-            }
+// Kernel variant of a block: slices per thread (NZ, ZS), channels per
+// staging thread (CPT: the smallest supported value whose window
+// W = tpr * CPT covers the tile footprint, 2 half_w + 3 channels) and the
+// staging threads per row (TPR: compile-time for the (TPR, CPT) shapes of
+// 16-, 32- and 64-row detectors at the usual fields of view, 0 = taken from
+// the arguments).  Encoded as one int; negative: a ctbb_status.
+constexpr int kCptList[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 14};
+
+bool has_static_tpr(int zs, int tpr, int cpt) {
+    const int key = tpr * 100 + cpt;
+    if (zs == 1) {
+        switch (key) {
+            case 406: case 408: case 410: case 412: case 414: case 804: case 806: case 808: case 1602: case 1604:
+                return true;
         }
-        switch (c) {
-            case 1: return launch_c<NZ, 1, 0, ZS>(a, stream);
-            case 2: return launch_c<NZ, 2, 0, ZS>(a, stream);
-            case 3: return launch_c<NZ, 3, 0, ZS>(a, stream);
-            case 4: return launch_c<NZ, 4, 0, ZS>(a, stream);
-            case 5: return launch_c<NZ, 5, 0, ZS>(a, stream);
-            case 6: return launch_c<NZ, 6, 0, ZS>(a, stream);
-            case 8: return launch_c<NZ, 8, 0, ZS>(a, stream);
-            case 10: return launch_c<NZ, 10, 0, ZS>(a, stream);
-            case 12: return launch_c<NZ, 12, 0, ZS>(a, stream);
-            case 14: return launch_c<NZ, 14, 0, ZS>(a, stream);
-        }
+        return false;
     }
-    return CTBB_ERR_SHAPE;   // footprint wider than 14 channels per thread
+    switch (key) {
+        case 804: case 805: case 806: case 1602: case 1603: case 3201: case 3202: return true;
+    }
+    return false;
+}
+
+int encode_variant(int nz, int cpt, int tpr_ct, int tpr, int bf16) {
+    return ((((bf16 * 64 + nz) * 32 + cpt) * 128 + tpr_ct) * 512 + tpr);
+}
+
+template <class Op>
+int with_static_tpr(int zs, int tpr, int cpt, Op op) {
+    if (zs == 1) {
+        switch (tpr * 100 + cpt) {
+            case 406: return op(std::integral_constant<int, 6>{}, std::integral_constant<int, 4>{});
+            case 408: return op(std::integral_constant<int, 8>{}, std::integral_constant<int, 4>{});
+            case 410: return op(std::integral_constant<int, 10>{}, std::integral_constant<int, 4>{});
+            case 412: return op(std::integral_constant<int, 12>{}, std::integral_constant<int, 4>{});
+            case 414: return op(std::integral_constant<int, 14>{}, std::integral_constant<int, 4>{});
+            case 804: return op(std::integral_constant<int, 4>{}, std::integral_constant<int, 8>{});
+            case 806: return op(std::integral_constant<int, 6>{}, std::integral_constant<int, 8>{});
+            case 808: return op(std::integral_constant<int, 8>{}, std::integral_constant<int, 8>{});
+            case 1602: return op(std::integral_constant<int, 2>{}, std::integral_constant<int, 16>{});
+            case 1604: return op(std::integral_constant<int, 4>{}, std::integral_constant<int, 16>{});
+        }
+        return CTBB_ERR_SHAPE;
+    }
+    switch (tpr * 100 + cpt) {
+        case 804: return op(std::integral_constant<int, 4>{}, std::integral_constant<int, 8>{});
+        case 805: return op(std::integral_constant<int, 5>{}, std::integral_constant<int, 8>{});
+        case 806: return op(std::integral_constant<int, 6>{}, std::integral_constant<int, 8>{});
+        case 1602: return op(std::integral_constant<int, 2>{}, std::integral_constant<int, 16>{});
+        case 1603: return op(std::integral_constant<int, 3>{}, std::integral_constant<int, 16>{});
+        case 3201: return op(std::integral_constant<int, 1>{}, std::integral_constant<int, 32>{});
+        case 3202: return op(std::integral_constant<int, 2>{}, std::integral_constant<int, 32>{});
+    }
+    return CTBB_ERR_SHAPE;
+}
+
+template <class Op>
+int with_generic(int cpt, Op op) {
+    switch (cpt) {
+        case 1: return op(std::integral_constant<int, 1>{}, std::integral_constant<int, 0>{});
+        case 2: return op(std::integral_constant<int, 2>{}, std::integral_constant<int, 0>{});
+        case 3: return op(std::integral_constant<int, 3>{}, std::integral_constant<int, 0>{});
+        case 4: return op(std::integral_constant<int, 4>{}, std::integral_constant<int, 0>{});
+        case 5: return op(std::integral_constant<int, 5>{}, std::integral_constant<int, 0>{});
+        case 6: return op(std::integral_constant<int, 6>{}, std::integral_constant<int, 0>{});
+        case 8: return op(std::integral_constant<int, 8>{}, std::integral_constant<int, 0>{});
+        case 10: return op(std::integral_constant<int, 10>{}, std::integral_constant<int, 0>{});
+        case 12: return op(std::integral_constant<int, 12>{}, std::integral_constant<int, 0>{});
+        case 14: return op(std::integral_constant<int, 14>{}, std::integral_constant<int, 0>{});
+    }
+    return CTBB_ERR_SHAPE;
+}
+
+// op(nz, zs, cpt, tpr) with compile-time constants for the variant id
+template <class Op>
+int with_variant(int id, Op op) {
+    const int tpr_ct = (id / 512) % 128, cpt = (id / (512 * 128)) % 32, nz = (id / (512 * 128 * 32)) % 64;
+    auto inner = [&](auto nzc, auto zsc) {
+        auto call = [&](auto cptc, auto tprc) { return op(nzc, zsc, cptc, tprc); };
+        return tpr_ct ? with_static_tpr(decltype(zsc)::value, tpr_ct, cpt, call) : with_generic(cpt, call);
+    };
+    if (nz == 32) return inner(std::integral_constant<int, 32>{}, std::integral_constant<int, 1>{});
+    if (nz == 16) return inner(std::integral_constant<int, 16>{}, std::integral_constant<int, 2>{});
+    return CTBB_ERR_ARG;
 }
 
 }  // namespace
 
-int launch_backproject(const BpArgs &a, hipStream_t stream) {
-    if ((!a.rebin && !a.rebin_bf16) || !a.vtab || !a.out) return CTBB_ERR_ARG;
-    if (a.nx % 16 || a.ny % 16 || a.nx <= 0 || a.ny <= 0) return CTBB_ERR_SHAPE;
-    if (a.n_rows < 2 || a.half <= 0 || a.n_ht <= 0) return CTBB_ERR_SHAPE;
-    if (a.n_tiles != (a.nx / 16) * (a.ny / 16) || a.tiles_y != a.ny / 16) return CTBB_ERR_ARG;
-    if (a.nz == 32) return launch_nz<32, 1>(a, stream);
-    if (a.nz == 16) return launch_nz<16, 2>(a, stream);
-    return CTBB_ERR_ARG;
+int backproject_variant(BpArgs &a) {
+    if (a.nx % 16 || a.ny % 16 || a.nx <= 0 || a.ny <= 0) return -CTBB_ERR_SHAPE;
+    if (a.n_rows < 2 || a.half <= 0 || a.n_ht <= 0) return -CTBB_ERR_SHAPE;
+    if (a.n_tiles != (a.nx / 16) * (a.ny / 16) || a.tiles_y != a.ny / 16) return -CTBB_ERR_ARG;
+    if (a.nz != 16 && a.nz != 32) return -CTBB_ERR_ARG;
+    const int zs = a.nz == 16 ? 2 : 1;
+    const int threads = 256 * zs;
+    int rows_pad = 2;
+    while (rows_pad < a.n_rows) rows_pad <<= 1;
+    if (rows_pad > threads) return -CTBB_ERR_SHAPE;
+    a.tpr = threads / rows_pad;
+    a.tpr_log2 = 0;
+    while ((1 << a.tpr_log2) < a.tpr) a.tpr_log2++;
+    const int need = static_cast<int>(ceilf((2.0f * a.half_w + 3.0f) / static_cast<float>(a.tpr)));
+    for (int c : kCptList) {
+        if (c < need) continue;
+        const int tpr_ct = has_static_tpr(zs, a.tpr, c) ? a.tpr : 0;
+        return encode_variant(a.nz, c, tpr_ct, a.tpr, a.rebin_bf16 ? 1 : 0);
+    }
+    return -CTBB_ERR_SHAPE;   // footprint wider than 14 channels per thread
+}
+
+int launch_backproject(const BpArgs &a0, hipStream_t stream) {
+    if ((!a0.rebin && !a0.rebin_bf16) || !a0.vtab || !a0.out) return CTBB_ERR_ARG;
+    BpArgs a = a0;
+    const int id = backproject_variant(a);
+    if (id < 0) return -id;
+    return with_variant(id, [&](auto nz, auto zs, auto cpt, auto tpr) {
+        return launch_c<decltype(nz)::value, decltype(cpt)::value, decltype(tpr)::value, decltype(zs)::value>(a,
+                                                                                                           stream);
+    });
+}
+
+int launch_backproject_persistent(const BpArgs &sample, int variant, const BpQueues &Q, int total_items,
+                                  int max_groups, hipStream_t stream) {
+    if (!Q.jobs || !Q.band_pre || !Q.counters || Q.n_jobs <= 0 || variant < 0) return CTBB_ERR_ARG;
+    if (total_items <= 0) return CTBB_OK;
+    return with_variant(variant, [&](auto nz, auto zs, auto cpt, auto tpr) {
+        return launch_p<decltype(nz)::value, decltype(cpt)::value, decltype(tpr)::value, decltype(zs)::value>(
+            sample, Q, total_items, max_groups, stream);
+    });
 }
 
 int launch_view_table(BpView *tab, double tube_start, int half, int n_ht, int n_proj_turn, double z_rot,

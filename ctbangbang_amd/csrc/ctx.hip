@@ -104,9 +104,18 @@ struct ctbb_ctx {
     std::map<std::string, std::vector<EventPair>> events;
     std::vector<EventPair> free_events;
     bool timing_failed = false;
+    // batching form: filtered-projection and image arenas per group parity,
+    // the device work list (plans, jobs, view tables, queues), and the
+    // executable graphs of launched group preparations (kept until the
+    // context's work has finished)
+    DevBuf arena[2], img_arena[2], plan_buf, tpos_buf;
+    hipEvent_t ev_gdone[2] = {nullptr, nullptr}, ev_gprep = nullptr;
+    std::vector<hipGraphExec_t> live_graphs;
+    bool capturing = false;      // a graph is being captured: no event timing
 
     EventPair begin(const char *family, hipStream_t on = nullptr) {
         EventPair e;
+        if (capturing) return EventPair{nullptr, nullptr, nullptr};
         if (!free_events.empty()) {
             e = free_events.back();
             free_events.pop_back();
@@ -687,6 +696,12 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     }
     if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
     if (c->stream_bp) (void)hipStreamDestroy(c->stream_bp);
+    for (hipGraphExec_t g : c->live_graphs) (void)hipGraphExecDestroy(g);
+    c->live_graphs.clear();
+    for (DevBuf *b : {&c->arena[0], &c->arena[1], &c->img_arena[0], &c->img_arena[1], &c->plan_buf, &c->tpos_buf})
+        b->release();
+    for (hipEvent_t e : {c->ev_gdone[0], c->ev_gdone[1], c->ev_gprep})
+        if (e) (void)hipEventDestroy(e);
     c->reset_events();
     for (auto &e : c->free_events) {
         (void)hipEventDestroy(e.a);
@@ -709,6 +724,8 @@ int ctbb_ctx_set_options(ctbb_ctx *c, const ctbb_options *o) {
     if (o->bp_tile != 0 && o->bp_tile != 16 && o->bp_tile != 32) return CTBB_ERR_ARG;
     if (o->bp_views_per_pass != 0 && o->bp_views_per_pass != 1) return CTBB_ERR_ARG;
     if (!(o->adaptive_threshold >= 0.0f && o->adaptive_threshold < 3.0e38f)) return CTBB_ERR_ARG;
+    if (o->batch_mode != CTBB_BATCH_PERSISTENT && o->batch_mode != CTBB_BATCH_PIPELINE) return CTBB_ERR_ARG;
+    if (o->batch_group_mb < 0) return CTBB_ERR_ARG;
     c->opt = *o;
     return CTBB_OK;
 }
@@ -719,6 +736,8 @@ int ctbb_ctx_synchronize(ctbb_ctx *c) {
     if (!c) return CTBB_ERR_ARG;
     if (c->stream_bp && hipStreamSynchronize(c->stream_bp) != hipSuccess) return CTBB_ERR_HIP;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return CTBB_ERR_HIP;
+    for (hipGraphExec_t g : c->live_graphs) (void)hipGraphExecDestroy(g);
+    c->live_graphs.clear();
     return take_nonfinite(c);
 }
 
@@ -1111,9 +1130,511 @@ int ctbb_reconstruct_host(ctbb_ctx *c, struct recon_metadata *mr, const float *h
     return take_nonfinite(c);
 }
 
+}  // extern "C"
+
+namespace {
+
+int batch_pipeline(ctbb_ctx *c, int n_scans, struct recon_metadata *const *mrs, const float *const *d_raws,
+                   const size_t *n_raw_readings, float *const *d_finals);
+
+// Copy host bytes to a device address and wait (the host buffer may be
+// reused when this returns).
+int upload_to(ctbb_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!bytes) return CTBB_OK;
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return CTBB_ERR_HIP;
+    return hip_ok(hipStreamSynchronize(ctx->stream));
+}
+
+// Raw pull of the current block of scan mr (mr->ri), padded with zeros where
+// it reaches outside the scan's readings [0, n_raw) (into ctx->raw_in).
+int block_window(ctbb_ctx *c, recon_metadata *m, const float *d_raw, size_t n_raw, const float **win) {
+    const size_t frame = m->cg.n_rows_raw * m->cg.n_channels;
+    const long start = static_cast<long>(static_cast<int>(m->ri.idx_pull_start));
+    const long n = static_cast<long>(m->ri.n_proj_pull);
+    if (start >= 0 && start + n <= static_cast<long>(n_raw)) {
+        *win = d_raw + static_cast<size_t>(start) * frame;
+        return CTBB_OK;
+    }
+    int st = c->raw_in.ensure(static_cast<size_t>(n) * frame * sizeof(float));
+    if (st) return st;
+    float *dst = c->raw_in.as<float>();
+    st = fill_zero(dst, static_cast<size_t>(n) * frame, c->stream);
+    if (st) return st;
+    long lo = start < 0 ? 0 : start, hi = start + n;
+    if (hi > static_cast<long>(n_raw)) hi = static_cast<long>(n_raw);
+    if (hi > lo && hipMemcpyAsync(dst + static_cast<size_t>(lo - start) * frame, d_raw + static_cast<size_t>(lo) * frame,
+                                  static_cast<size_t>(hi - lo) * frame * sizeof(float), hipMemcpyDeviceToDevice,
+                                  c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    *win = dst;
+    return CTBB_OK;
+}
+
+// Views the preparation rebins and filters for the current block (see
+// prep_block): the backprojection's views plus, for an odd row count, the
+// FFT row-pair partner.
+size_t prep_views(const recon_metadata *m) {
+    ctbb_sizes sz;
+    ctbb_block_sizes(m, &sz);
+    size_t views = std::min(sz.n_proj_out, static_cast<size_t>(m->cg.n_proj_turn / 2) *
+                                               static_cast<size_t>(std::max(bp_n_half_turns(*m), 0)));
+    if (((views * m->cg.n_rows) & 1) && views < sz.n_proj_out) views += 1;
+    return views;
+}
+
+// Planner input of scan m (table positions at d_tpos, its blocks numbered
+// from block_first in the planner's output); image and filtered-block
+// pointers are set by the caller.
+ScanPlanIn scan_plan_in(const recon_metadata *m, const BpArgs &tmpl, const double *d_tpos, int block_first) {
+    ScanPlanIn in{};
+    in.table_positions = d_tpos;
+    in.tmpl = tmpl;
+    in.recon_start_pos = m->ri.recon_start_pos;
+    in.coll_slicewidth = m->rp.coll_slicewidth;
+    in.tube_angle0 = m->tube_angles[0];
+    in.z_rot = m->cg.z_rot;
+    in.n_readings = static_cast<int>(m->rp.n_readings);
+    // direction_of / recon_direction_of of the host planner (setup.cpp)
+    const double dt = m->table_positions[100] - m->table_positions[0];
+    in.array_direction = dt == 0.0 ? 1 : (dt > 0.0 ? 1 : -1);
+    const float dpos = m->rp.end_pos - m->rp.start_pos;
+    in.recon_direction = dpos == 0.0f ? 1 : (dpos > 0.0f ? 1 : -1);
+    in.n_blocks = static_cast<int>(m->ri.n_blocks);
+    in.n_slices_block = static_cast<int>(m->ri.n_slices_block);
+    in.block_first = block_first;
+    in.n_ffs = static_cast<int>(std::pow(2, m->rp.z_ffs) * std::pow(2, m->rp.phi_ffs));
+    in.ppb = static_cast<int>(m->cg.n_proj_ffs / 2);
+    in.add_ffs = static_cast<int>(m->cg.add_projections_ffs);
+    in.n_proj_ffs = static_cast<int>(m->cg.n_proj_ffs);
+    in.add_projections = static_cast<int>(m->cg.add_projections);
+    in.n_proj_turn = static_cast<int>(m->cg.n_proj_turn);
+    in.add_projections_ffs = m->cg.add_projections_ffs;
+    return in;
+}
+
+struct BatchScan {
+    int s = 0, variant = 0, n_blocks = 0, views_max = 0;
+    size_t slot_elems = 0, img_elems = 0, pull_max = 0, rebin_max = 0, views_total = 0;
+    BpArgs tmpl{};
+};
+
+struct BatchGroup {
+    std::vector<int> scans;       // indices into the batch's scan list, one kernel variant
+    size_t slot_elems = 0, arena_elems = 0, img_elems = 0;
+    int n_blocks = 0, views_max = 0;
+    size_t views_total = 0;
+};
+
+// Persistent batching form (see ctbb_reconstruct_batch_dev).
+int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const float *const *d_raws,
+                     const size_t *n_raw, float *const *d_finals) {
+    const size_t eb = proj_elem_bytes(c);
+    std::vector<BatchScan> info(n_scans);
+    size_t pull_max = 0, rebin_max = 0;
+    for (int s = 0; s < n_scans; s++) {
+        recon_metadata *m = mrs[s];
+        BatchScan &bs = info[s];
+        bs.s = s;
+        bs.n_blocks = static_cast<int>(m->ri.n_blocks);
+        for (int b = 0; b < bs.n_blocks; b++) {
+            ctbb::plan_block(m, b);
+            int st = check_geometry(m);
+            if (st) return st;
+            if (bp_n_half_turns(*m) <= 0) return CTBB_ERR_SHAPE;
+            ctbb_sizes sz;
+            ctbb_block_sizes(m, &sz);
+            const size_t views = prep_views(m);
+            bs.slot_elems = std::max(bs.slot_elems, views * m->cg.n_rows * m->cg.n_channels_oversampled);
+            bs.views_max = std::max(bs.views_max, static_cast<int>(m->cg.n_proj_turn / 2) * bp_n_half_turns(*m));
+            bs.views_total += static_cast<size_t>(m->cg.n_proj_turn / 2) * bp_n_half_turns(*m);
+            pull_max = std::max(pull_max, static_cast<size_t>(m->ri.n_proj_pull) * m->cg.n_rows_raw * m->cg.n_channels);
+            rebin_max = std::max(rebin_max, sz.rebin);
+        }
+        bs.slot_elems = (bs.slot_elems + 63) & ~static_cast<size_t>(63);      // 256-byte aligned slots
+        bs.img_elems = static_cast<size_t>(m->ri.n_slices_recon) * m->rp.nx * m->rp.ny;
+        int st = fill_bp_args(m, bs.tmpl);
+        if (st) return st;
+        bs.tmpl.nz = c->opt.bp_tile == 32 ? 32 : 16;
+        if (c->opt.proj_dtype == CTBB_PROJ_BF16) bs.tmpl.rebin_bf16 = reinterpret_cast<const uint16_t *>(16);
+        else bs.tmpl.rebin = reinterpret_cast<const float *>(16);   // placeholders for the variant choice
+        bs.tmpl.vtab = reinterpret_cast<const BpView *>(16);
+        bs.tmpl.out = reinterpret_cast<float *>(16);
+        bs.variant = backproject_variant(bs.tmpl);
+        if (bs.variant < 0) return -bs.variant;
+    }
+    // groups: scans of one kernel variant (in batch order within it) whose
+    // filtered blocks and images fit the budget (a larger scan is a group of
+    // its own)
+    const size_t budget = (c->opt.batch_group_mb > 0 ? static_cast<size_t>(c->opt.batch_group_mb) : 8192) << 20;
+    std::vector<int> order(n_scans);
+    for (int s = 0; s < n_scans; s++) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return info[a].variant < info[b].variant; });
+    std::vector<BatchGroup> groups;
+    for (int s : order) {
+        const BatchScan &bs = info[s];
+        const size_t bytes = bs.slot_elems * bs.n_blocks * eb + bs.img_elems * sizeof(float);
+        BatchGroup *g = groups.empty() ? nullptr : &groups.back();
+        if (!g || info[g->scans.front()].variant != bs.variant ||
+            (g->arena_elems * eb + g->img_elems * sizeof(float) + bytes > budget)) {
+            groups.emplace_back();
+            g = &groups.back();
+        }
+        g->scans.push_back(s);
+        g->arena_elems += bs.slot_elems * bs.n_blocks;
+        g->img_elems += bs.img_elems;
+        g->n_blocks += bs.n_blocks;
+        g->views_max = std::max(g->views_max, bs.views_max);
+        g->views_total += bs.views_total;
+    }
+    // the first group's preparation overlaps nothing: start with the smallest
+    if (groups.size() > 1) {
+        auto smallest = std::min_element(groups.begin(), groups.end(), [](const BatchGroup &x, const BatchGroup &y) {
+            return x.n_blocks < y.n_blocks;
+        });
+        std::rotate(groups.begin(), smallest, smallest + 1);
+    }
+    size_t arena_max = 0, img_max = 0, views_all = 0;
+    int blocks_all = 0;
+    for (const BatchGroup &g : groups) {
+        arena_max = std::max(arena_max, g.arena_elems);
+        img_max = std::max(img_max, g.img_elems);
+        views_all += g.views_total;
+        blocks_all += g.n_blocks;
+    }
+    // device memory: arenas, preparation workspaces, view tables
+    int st;
+    for (int k = 0; k < 2 && k < static_cast<int>(groups.size()); k++) {
+        if ((st = c->arena[k].ensure(arena_max * eb)) || (st = c->img_arena[k].ensure(img_max * sizeof(float))))
+            return st;
+    }
+    if ((st = c->raw_af.ensure(pull_max * sizeof(float))) || (st = c->raw_in.ensure(pull_max * sizeof(float))) ||
+        (st = c->rebin_tmp.ensure(rebin_max * sizeof(float))))
+        return st;
+    size_t np_max = 0;
+    for (int s = 0; s < n_scans; s++)
+        for (int b = 0; b < info[s].n_blocks; b++) {
+            ctbb::plan_block(mrs[s], b);
+            np_max = std::max(np_max, static_cast<size_t>(mrs[s]->ri.n_proj_pull));
+        }
+    if ((st = c->af_work.ensure(6 * np_max * sizeof(float)))) return st;
+    if ((st = c->vtab.ensure(views_all * sizeof(BpView)))) return st;
+    if ((st = ensure_pipeline(c, 0))) return st;
+    for (hipEvent_t *e : {&c->ev_gdone[0], &c->ev_gdone[1], &c->ev_gprep})
+        if (!*e && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return CTBB_ERR_HIP;
+    // host-built inputs of the device planner, one upload: per scan its
+    // ScanPlanIn (in group order) and table positions; per group the plans,
+    // jobs, view-table offsets, queue prefixes and counters
+    size_t tpos_total = 0;
+    std::vector<size_t> tpos_off(n_scans);
+    for (int s = 0; s < n_scans; s++) {
+        tpos_off[s] = tpos_total;
+        tpos_total += static_cast<size_t>(mrs[s]->rp.n_readings);
+    }
+    if ((st = c->tpos_buf.ensure(tpos_total * sizeof(double)))) return st;
+    auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+    struct GroupLayout {
+        size_t scans, plans, jobs, vt_off, band_pre, counters, vtab;
+    };
+    std::vector<GroupLayout> lay(groups.size());
+    size_t bytes = 0, vtab_at = 0;
+    for (size_t g = 0; g < groups.size(); g++) {
+        const int nb = groups[g].n_blocks;
+        lay[g].scans = bytes;
+        bytes = align(bytes + groups[g].scans.size() * sizeof(ScanPlanIn));
+        lay[g].plans = bytes;
+        bytes = align(bytes + nb * sizeof(BlockPlan));
+        lay[g].jobs = bytes;
+        bytes = align(bytes + nb * sizeof(BpArgs));
+        lay[g].vt_off = bytes;
+        bytes = align(bytes + (nb + 1) * sizeof(size_t));
+        lay[g].band_pre = bytes;
+        bytes = align(bytes + 8 * (nb + 1) * sizeof(int));
+        lay[g].counters = bytes;
+        bytes = align(bytes + 8 * sizeof(int));
+        lay[g].vtab = vtab_at;
+        vtab_at += groups[g].views_total;
+    }
+    if ((st = c->plan_buf.ensure(bytes))) return st;
+    std::vector<char> host(bytes, 0);
+    char *dplan = c->plan_buf.as<char>();
+    for (size_t g = 0; g < groups.size(); g++) {
+        const int k = static_cast<int>(g & 1);
+        size_t slot_at = 0, img_at = 0;
+        int block_first = 0;
+        for (size_t i = 0; i < groups[g].scans.size(); i++) {
+            const int s = groups[g].scans[i];
+            const recon_metadata *m = mrs[s];
+            const BatchScan &bs = info[s];
+            ScanPlanIn in = scan_plan_in(m, bs.tmpl, c->tpos_buf.as<double>() + tpos_off[s], block_first);
+            in.image = c->img_arena[k].as<float>() + img_at;
+            if (c->opt.proj_dtype == CTBB_PROJ_BF16) {
+                in.tmpl.rebin_bf16 = c->arena[k].as<uint16_t>() + slot_at;
+                in.tmpl.rebin = nullptr;
+            } else {
+                in.tmpl.rebin = c->arena[k].as<float>() + slot_at;
+                in.tmpl.rebin_bf16 = nullptr;
+            }
+            in.slot_elems = bs.slot_elems;
+            std::memcpy(host.data() + lay[g].scans + i * sizeof(ScanPlanIn), &in, sizeof(in));
+            slot_at += bs.slot_elems * bs.n_blocks;
+            img_at += bs.img_elems;
+            block_first += bs.n_blocks;
+        }
+    }
+    if ((st = upload_to(c, c->plan_buf.p, host.data(), bytes))) return st;
+    for (int s = 0; s < n_scans; s++)
+        if ((st = upload_to(c, c->tpos_buf.as<double>() + tpos_off[s], mrs[s]->table_positions,
+                            static_cast<size_t>(mrs[s]->rp.n_readings) * sizeof(double))))
+            return st;
+    // finalize tap lists of every scan
+    std::vector<char> fin;
+    std::vector<size_t> fin_off(n_scans);
+    std::vector<int> fin_mt(n_scans), fin_nf(n_scans);
+    for (int s = 0; s < n_scans; s++) {
+        std::vector<char> blob;
+        if ((st = finalize_blob(mrs[s], blob, &fin_mt[s], &fin_nf[s]))) return st;
+        fin_off[s] = align(fin.size());
+        fin.resize(fin_off[s] + blob.size());
+        std::memcpy(fin.data() + fin_off[s], blob.data(), blob.size());
+    }
+    if ((st = upload(c, c->fin_tab, fin.data(), fin.size()))) return st;
+    // the device work list of every group: plans, jobs, view tables, queues;
+    // all queue counters start at zero
+    for (size_t g = 0; g < groups.size(); g++) {
+        st = launch_plan_batch(reinterpret_cast<const ScanPlanIn *>(dplan + lay[g].scans),
+                               static_cast<int>(groups[g].scans.size()), groups[g].n_blocks, groups[g].views_max,
+                               reinterpret_cast<BlockPlan *>(dplan + lay[g].plans),
+                               reinterpret_cast<BpArgs *>(dplan + lay[g].jobs), c->vtab.as<BpView>() + lay[g].vtab,
+                               reinterpret_cast<size_t *>(dplan + lay[g].vt_off),
+                               reinterpret_cast<int *>(dplan + lay[g].band_pre), c->stream);
+        if (st) return st;
+        st = fill_zero(reinterpret_cast<float *>(dplan + lay[g].counters), 8, c->stream);
+        if (st) return st;
+    }
+    // the backprojection stream starts after the work list
+    if (hipEventRecord(c->ev_sync, c->stream) != hipSuccess || hipStreamWaitEvent(c->stream_bp, c->ev_sync, 0) != hipSuccess)
+        return CTBB_ERR_HIP;
+    const bool split = c->opt.proj_dtype != CTBB_PROJ_BF16 && c->opt.filter_mode == CTBB_FILTER_FFT32;
+    for (size_t g = 0; g < groups.size(); g++) {
+        const int k = static_cast<int>(g & 1);
+        // preparation into arena k once the group that used it before (g - 2)
+        // is backprojected and finalized
+        if (g >= 2 && hipStreamWaitEvent(c->stream, c->ev_gdone[k], 0) != hipSuccess) return CTBB_ERR_HIP;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return CTBB_ERR_HIP;
+        c->capturing = true;
+        size_t slot_at = 0;
+        bool first = g == 0;
+        if (split) {
+            // FP32 projections, FFT filter: first the adaptive filter and the
+            // rebinning of every block, straight into its slot (low-register
+            // kernels that run beside the previous group's backprojection),
+            // then the filter of every slot in place (the FFT kernel needs
+            // the registers the backprojection holds)
+            for (int s : groups[g].scans) {
+                recon_metadata *m = mrs[s];
+                for (int b = 0; b < info[s].n_blocks && !st; b++) {
+                    ctbb::plan_block(m, b);
+                    const float *win = nullptr;
+                    st = block_window(c, m, d_raws[s], n_raw[s], &win);
+                    if (!st && c->opt.adaptive_filter) {
+                        st = do_adaptive(c, m, win, c->raw_af.as<float>(), first);
+                        win = c->raw_af.as<float>();
+                    }
+                    float *slot = c->arena[k].as<float>() + slot_at + b * info[s].slot_elems;
+                    if (!st) st = do_rebin(c, m, win, slot, static_cast<int>(prep_views(m)));
+                    first = false;
+                }
+                slot_at += info[s].slot_elems * info[s].n_blocks;
+                if (st) break;
+            }
+            slot_at = 0;
+            for (int s : groups[g].scans) {
+                recon_metadata *m = mrs[s];
+                for (int b = 0; b < info[s].n_blocks && !st; b++) {
+                    ctbb::plan_block(m, b);
+                    float *slot = c->arena[k].as<float>() + slot_at + b * info[s].slot_elems;
+                    st = do_filter(c, c->taps.as<float>(), static_cast<int>(m->cg.n_channels_oversampled),
+                                   prep_views(m) * m->cg.n_rows, slot, slot, 0);
+                }
+                slot_at += info[s].slot_elems * info[s].n_blocks;
+                if (st) break;
+            }
+        } else {
+            for (int s : groups[g].scans) {
+                recon_metadata *m = mrs[s];
+                for (int b = 0; b < info[s].n_blocks && !st; b++) {
+                    ctbb::plan_block(m, b);
+                    const float *win = nullptr;
+                    st = block_window(c, m, d_raws[s], n_raw[s], &win);
+                    if (!st)
+                        st = prep_block(c, m, win, c->arena[k].as<char>() + (slot_at + b * info[s].slot_elems) * eb,
+                                        first);
+                    first = false;
+                }
+                slot_at += info[s].slot_elems * info[s].n_blocks;
+                if (st) break;
+            }
+        }
+        c->capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (st || ec != hipSuccess || !graph) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st ? st : CTBB_ERR_HIP;
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess || !exec) return CTBB_ERR_HIP;
+        c->live_graphs.push_back(exec);
+        EventPair e = c->begin("prepare", c->stream);
+        if (hipGraphLaunch(exec, c->stream) != hipSuccess) return CTBB_ERR_HIP;
+        c->end(e);
+        if (hipEventRecord(c->ev_gprep, c->stream) != hipSuccess ||
+            hipStreamWaitEvent(c->stream_bp, c->ev_gprep, 0) != hipSuccess)
+            return CTBB_ERR_HIP;
+        // one persistent backprojection over every (block, tile) of the group
+        BpQueues Q{};
+        Q.jobs = reinterpret_cast<const BpArgs *>(dplan + lay[g].jobs);
+        Q.band_pre = reinterpret_cast<const int *>(dplan + lay[g].band_pre);
+        Q.counters = reinterpret_cast<int *>(dplan + lay[g].counters);
+        Q.n_jobs = groups[g].n_blocks;
+        const BatchScan &s0 = info[groups[g].scans.front()];
+        int total = 0;
+        for (int s : groups[g].scans) total += info[s].n_blocks * info[s].tmpl.n_tiles;
+        EventPair eb2 = c->begin("backproject", c->stream_bp);
+        st = launch_backproject_persistent(s0.tmpl, s0.variant, Q, total, 0, c->stream_bp);
+        c->end(eb2);
+        if (st) return st;
+        // finalize every scan of the group behind it
+        size_t img_at = 0;
+        for (int s : groups[g].scans) {
+            st = launch_finalize_blob(c, mrs[s], c->img_arena[k].as<float>() + img_at, d_finals[s],
+                                      c->fin_tab.as<char>() + fin_off[s], fin_mt[s], fin_nf[s], c->stream_bp);
+            if (st) return st;
+            img_at += info[s].img_elems;
+        }
+        if (hipEventRecord(c->ev_gdone[k], c->stream_bp) != hipSuccess) return CTBB_ERR_HIP;
+    }
+    return join_pipeline(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctbb_plan_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *const *mrs, ctbb_block_plan *out,
+                        int max_out) {
+    if (!c || n_scans <= 0 || !mrs || !out) return CTBB_ERR_ARG;
+    int n_blocks = 0;
+    for (int s = 0; s < n_scans; s++) {
+        const recon_metadata *m = mrs[s];
+        if (!m || !m->table_positions || !m->tube_angles || m->ri.n_blocks <= 0 || m->rp.n_readings <= 100)
+            return CTBB_ERR_ARG;
+        n_blocks += static_cast<int>(m->ri.n_blocks);
+    }
+    if (n_blocks > max_out) return CTBB_ERR_LIMIT;
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
+    // planner inputs (geometry templates are not needed for the plans)
+    size_t tpos_total = 0;
+    std::vector<size_t> tpos_off(n_scans);
+    for (int s = 0; s < n_scans; s++) {
+        tpos_off[s] = tpos_total;
+        tpos_total += static_cast<size_t>(mrs[s]->rp.n_readings);
+    }
+    int st;
+    DevBuf tpos, work;
+    const size_t b_scans = n_scans * sizeof(ScanPlanIn), b_plans = n_blocks * sizeof(BlockPlan),
+                 b_jobs = n_blocks * sizeof(BpArgs), b_off = (n_blocks + 1) * sizeof(size_t),
+                 b_pre = 8 * (n_blocks + 1) * sizeof(int);
+    if ((st = tpos.ensure(tpos_total * sizeof(double))) ||
+        (st = work.ensure(b_scans + b_plans + b_jobs + b_off + b_pre + 4 * 256)))
+        return st;
+    char *w = work.as<char>();
+    std::vector<ScanPlanIn> ins;
+    int first = 0;
+    for (int s = 0; s < n_scans; s++) {
+        BpArgs t{};
+        t.n_tiles = 0;
+        ins.push_back(scan_plan_in(mrs[s], t, tpos.as<double>() + tpos_off[s], first));
+        first += static_cast<int>(mrs[s]->ri.n_blocks);
+    }
+    st = upload_to(c, w, ins.data(), b_scans);
+    for (int s = 0; s < n_scans && !st; s++)
+        st = upload_to(c, tpos.as<double>() + tpos_off[s], mrs[s]->table_positions,
+                       static_cast<size_t>(mrs[s]->rp.n_readings) * sizeof(double));
+    if (st) return st;
+    char *p_plans = w + ((b_scans + 255) & ~size_t(255));
+    char *p_jobs = p_plans + ((b_plans + 255) & ~size_t(255));
+    char *p_off = p_jobs + ((b_jobs + 255) & ~size_t(255));
+    char *p_pre = p_off + ((b_off + 255) & ~size_t(255));
+    // (no view tables: max_views_per_block 0; the table base only offsets
+    // the jobs' pointers, which are not read back)
+    st = launch_plan_batch(reinterpret_cast<const ScanPlanIn *>(w), n_scans, n_blocks, 0,
+                           reinterpret_cast<BlockPlan *>(p_plans), reinterpret_cast<BpArgs *>(p_jobs),
+                           reinterpret_cast<BpView *>(p_jobs), reinterpret_cast<size_t *>(p_off),
+                           reinterpret_cast<int *>(p_pre), c->stream);
+    if (st) return st;
+    std::vector<BlockPlan> plans(n_blocks);
+    if (hipMemcpyAsync(plans.data(), p_plans, b_plans, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return CTBB_ERR_HIP;
+    for (int j = 0; j < n_blocks; j++) {
+        ctbb_block_plan &o = out[j];
+        o.scan = plans[j].scan;
+        o.block = plans[j].block;
+        o.idx_pull_start = plans[j].pull_start;
+        o.n_proj_pull = plans[j].n_pull;
+        o.n_half_turns = plans[j].n_ht;
+        o.idx_block_slice_start = plans[j].idx_slice_start;
+        o.idx_block_slice_end = plans[j].idx_slice_end;
+        o.block_slice_start = plans[j].slice_start;
+        o.block_slice_end = plans[j].slice_end;
+        o.tube_start = plans[j].tube_start;
+    }
+    return CTBB_OK;
+}
+
 int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *const *mrs,
                                const float *const *d_raws, const size_t *n_raw_readings, float *const *d_finals) {
     if (!c || n_scans <= 0 || !mrs || !d_raws || !n_raw_readings || !d_finals) return CTBB_ERR_ARG;
+    for (int s = 0; s < n_scans; s++) {
+        const recon_metadata *m = mrs[s];
+        if (!m || !d_raws[s] || !d_finals[s] || !m->table_positions || !m->tube_angles || m->ri.n_blocks <= 0 ||
+            m->rp.n_readings <= 100)
+            return CTBB_ERR_ARG;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
+    if (c->opt.batch_mode == CTBB_BATCH_PIPELINE) return batch_pipeline(c, n_scans, mrs, d_raws, n_raw_readings, d_finals);
+    int st = ensure_taps(c, mrs[0]);
+    if (st) return st;
+    st = ensure_maps(c, mrs[0]);
+    if (st) return st;
+    const std::string taps0 = c->taps_key, maps0 = c->maps_key;
+    for (int s = 1; s < n_scans; s++) {
+        if ((st = ensure_taps(c, mrs[s])) || (st = ensure_maps(c, mrs[s]))) return st;
+        if (c->taps_key != taps0 || c->maps_key != maps0) {
+            // different scanners, kernels or focal-spot modes: one scan at a time
+            for (int t = 0; t < n_scans; t++) {
+                st = ctbb_reconstruct_dev(c, mrs[t], d_raws[t], n_raw_readings[t], nullptr, d_finals[t]);
+                if (st) return st;
+            }
+            return CTBB_OK;
+        }
+    }
+    c->reset_events();
+    return batch_persistent(c, n_scans, mrs, d_raws, n_raw_readings, d_finals);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Per-block pipeline form of the batch (batch_mode CTBB_BATCH_PIPELINE): one
+// launch per block, block preparation overlapping the previous block's
+// backprojection.
+int batch_pipeline(ctbb_ctx *c, int n_scans, struct recon_metadata *const *mrs, const float *const *d_raws,
+                   const size_t *n_raw_readings, float *const *d_finals) {
     for (int s = 0; s < n_scans; s++) {
         const recon_metadata *m = mrs[s];
         if (!m || !d_raws[s] || !d_finals[s] || !m->table_positions || !m->tube_angles || m->ri.n_blocks <= 0)
@@ -1226,6 +1747,10 @@ int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *
     }
     return join_pipeline(c);
 }
+
+}  // namespace
+
+extern "C" {
 
 // --- reference host-memory contract ---------------------------------------
 

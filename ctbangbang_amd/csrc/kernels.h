@@ -33,6 +33,25 @@ struct BpArgs {
 };
 int launch_backproject(const BpArgs &a, hipStream_t stream);
 
+// Persistent backprojection over the tiles of many blocks (jobs).  The jobs
+// of one launch share the kernel variant (rows, staging shape, slices per
+// thread, projection type): see backproject_variant.
+struct BpQueues {
+    const BpArgs *jobs;        // [n_jobs] (device memory)
+    const int *band_pre;       // [8][n_jobs + 1]: items of queue q before job j (band q of each job)
+    int *counters;             // [8], zero at launch
+    int n_jobs;
+};
+// Fills the staging shape (tpr, tpr_log2) of a and returns an identifier of
+// its kernel variant (equal identifiers: one persistent launch may hold
+// both), or a negative ctbb_status.
+int backproject_variant(BpArgs &a);
+// Work-queue launch over the jobs of Q (all of variant `variant`, job
+// arguments as the variant filled them); `sample` is any of those jobs
+// (host copy).  max_groups bounds the grid (0: the resident workgroups).
+int launch_backproject_persistent(const BpArgs &sample, int variant, const BpQueues &Q, int total_items,
+                                  int max_groups, hipStream_t stream);
+
 // ---------------------------------------------------------------------------
 // Rebinning (fused single pass per FFS mode)
 struct RebinArgs {
@@ -90,6 +109,51 @@ int launch_finalize(const FinalizeArgs &a, hipStream_t stream);
 
 // Backprojection view table of one block (half * n_ht entries) on the device
 int launch_view_table(BpView *tab, double tube_start, int half, int n_ht, int n_proj_turn, double z_rot,
+                      hipStream_t stream);
+
+// Entry e = i * n_ht + k (angle i < half, half turn k < n_ht) of a block's
+// view table: theta = tube_start + (2 pi' / n_proj_turn) i + k pi'
+// (reference include/backproject.cuh:75), its sine and cosine, the z term
+// z_rot / 2 - z_rot (theta - tube_start) / (2 pi') and the rebinned
+// projection i + k * half, in FP64 rounded to FP32 once.  Explicitly rounded
+// operations keep the FP64 expression free of contractions.
+__device__ inline void view_table_entry(BpView *out, int e, double tube_start, int half, int n_ht, int n_proj_turn,
+                                        double z_rot) {
+    const int i = e / n_ht, k = e - i * n_ht;
+    const double pi = static_cast<double>(kPiBp);
+    const double dtheta = __dadd_rn(__dmul_rn(__ddiv_rn(2.0 * pi, static_cast<double>(n_proj_turn)), i),
+                                    __dmul_rn(static_cast<double>(k), pi));
+    const double th = __dadd_rn(tube_start, dtheta);
+    BpView v;
+    v.sin_t = static_cast<float>(sin(th));
+    v.cos_t = static_cast<float>(cos(th));
+    v.c_v = static_cast<float>(__dsub_rn(z_rot / 2.0, __ddiv_rn(__dmul_rn(z_rot, dtheta), 2.0 * pi)));
+    v.proj = i + k * half;
+    *out = v;
+}
+
+// ---------------------------------------------------------------------------
+// Batching form: the work list built on the device (plan.hip)
+struct ScanPlanIn {
+    const double *table_positions;  // device copy, n_readings entries
+    float *image;                   // the scan's image stack [n_slices_recon][nx][ny]
+    BpArgs tmpl;                    // geometry of the scan's jobs; rebin / rebin_bf16 = its first filtered block
+    size_t slot_elems;              // elements between consecutive filtered blocks of the scan
+    float recon_start_pos, coll_slicewidth, tube_angle0, z_rot;
+    int n_readings, array_direction, recon_direction, n_blocks, n_slices_block, block_first;
+    int n_ffs, ppb, add_ffs, n_proj_ffs, add_projections, n_proj_turn;
+    size_t add_projections_ffs;
+};
+struct BlockPlan {
+    int scan, block;
+    float slice_start, slice_end, tube_start;
+    int idx_slice_start, idx_slice_end, pull_start, pull_end, n_pull, n_ht;
+};
+// plans[n_blocks], jobs[n_blocks] (jobs' geometry from their scan's tmpl),
+// view tables of every block at d_vtab + vt_off[block] (vt_off[n_blocks + 1]),
+// queue prefixes band_pre[8][n_blocks + 1].
+int launch_plan_batch(const ScanPlanIn *d_scans, int n_scans, int n_blocks, int max_views_per_block,
+                      BlockPlan *d_plans, BpArgs *d_jobs, BpView *d_vtab, size_t *d_vt_off, int *d_band_pre,
                       hipStream_t stream);
 
 // HU -> int16 stored pixel values (HU = slope * stored + intercept)

@@ -154,6 +154,8 @@ class Options(C.Structure):
         ("bp_views_per_pass", C.c_int),
         ("adaptive_filter", C.c_int),
         ("adaptive_threshold", C.c_float),
+        ("batch_mode", C.c_int),
+        ("batch_group_mb", C.c_int),
     ]
 
 

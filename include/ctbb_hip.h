@@ -138,6 +138,12 @@ typedef enum {
     CTBB_PROJ_BF16 = 1          /* filtered projections stored as bfloat16 */
 } ctbb_proj_dtype;
 
+typedef enum {
+    CTBB_BATCH_PERSISTENT = 0,  /* groups of scans: captured preparation graph + one persistent
+                                   backprojection launch per group (default) */
+    CTBB_BATCH_PIPELINE = 1     /* one launch per block, preparation beside the previous block */
+} ctbb_batch_mode;
+
 typedef struct ctbb_options_s {
     int filter_mode;            /* ctbb_filter_mode; default CTBB_FILTER_FFT32 */
     int proj_dtype;             /* ctbb_proj_dtype */
@@ -151,6 +157,9 @@ typedef struct ctbb_options_s {
                                    line-integral threshold, e.g. ln(I0 / N) for a noise threshold
                                    of N photons at I0 incident photons (no counterpart in the
                                    reference) */
+    int batch_mode;             /* ctbb_batch_mode of ctbb_reconstruct_batch_dev */
+    int batch_group_mb;         /* device memory of one batch group (filtered projections and
+                                   images), MiB; 0: 8192 */
 } ctbb_options;
 
 /* Create a context on HIP device `device` (-1: current device).  The context
@@ -325,6 +334,21 @@ int ctbb_reconstruct_host(ctbb_ctx *ctx, struct recon_metadata *mr,
 int ctbb_reconstruct_batch_dev(ctbb_ctx *ctx, int n_scans, struct recon_metadata *const *mrs,
                                const float *const *d_raws, const size_t *n_raw_readings,
                                float *const *d_finals);
+
+/* Block plans of a batch, computed on the device the way the batching form
+ * builds its work list (the arithmetic of update_block_info, reference
+ * src/setup.cu:605-679, and of the backprojection's tube start and half
+ * turns, src/backproject.cu:37-39): one entry per (scan, block) in batch
+ * order, into out[0 .. total blocks).  CTBB_ERR_LIMIT if they exceed
+ * max_out.  For checking the device planner against ctbb_plan_block. */
+typedef struct {
+    int scan, block;
+    int idx_pull_start, n_proj_pull, n_half_turns;
+    int idx_block_slice_start, idx_block_slice_end;
+    float block_slice_start, block_slice_end, tube_start;
+} ctbb_block_plan;
+int ctbb_plan_batch_dev(ctbb_ctx *ctx, int n_scans, struct recon_metadata *const *mrs, ctbb_block_plan *out,
+                        int max_out);
 
 /* ------------------------------------------------------------------------ */
 /* CPU reconstruction path (the reference driver's --no-gpu mode).           */

@@ -464,10 +464,15 @@ def test_reconstruct_host_matches_dev(ctx, tmp_path, mini_dir, ffs):
     cb.free_metadata(mr)
 
 
-def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir):
+@pytest.mark.parametrize("mode,group_mb", [(cb.BATCH_PERSISTENT, 0), (cb.BATCH_PERSISTENT, 1),
+                                           (cb.BATCH_PIPELINE, 0)])
+def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir, mode, group_mb):
     """Batching form: scans of different length, pitch, FOV and matrix size
-    through one pipeline give exactly the one-scan-at-a-time results; a batch
-    mixing focal-spot modes (other rebin maps) falls back and still agrees."""
+    give exactly the one-scan-at-a-time results, through the persistent form
+    (scans grouped per kernel variant; with a 1 MiB group budget every scan
+    is a group of its own, so the two arenas alternate) and through the
+    per-block pipeline; a batch mixing focal-spot modes (other rebin maps)
+    falls back and still agrees."""
     def scan(ffs, nx, rot, n_sl, fov, pitch, name):
         tok = scans.mini_tokens(ffs, nrows=8, nx=nx, rotations=rot, n_slices=n_sl)
         tok["ReconFOV"] = fov
@@ -491,9 +496,11 @@ def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir):
             singles.append(host(out).copy())
             out.zero_()
         torch.cuda.synchronize()    # torch's zero fill before the context's streams write out
+        ctx.set_options(batch_mode=mode, batch_group_mb=group_mb)
         ctx.reconstruct_batch_dev([i[0] for i in items], [i[1] for i in items], [i[2] for i in items],
                                   [i[3] for i in items])
         ctx.synchronize()
+        ctx.set_options(batch_mode=cb.BATCH_PERSISTENT, batch_group_mb=0)
         for (mr, raw, n, out), want in zip(items, singles):
             assert np.array_equal(host(out), want)
             cb.free_metadata(mr)
@@ -825,3 +832,36 @@ def test_end_to_end_fixed_threshold(ctx, tmp_path, mini_dir):
     assert smoothed > 0
     assert rms <= 0.05 and mx <= 2.0
     cb.free_metadata(mr)
+
+
+def test_device_planner_matches_host(ctx, tmp_path):
+    """The batching form's work list is planned on the device: every block
+    plan of the 32 seeded batch scans and of the c1 scan (pull range, slice
+    range and its table-position indices, half turns) equals the host
+    planner's (update_block_info arithmetic)."""
+    mrs = []
+    for i in range(32):
+        mrs.append(_mr(tmp_path, scans.c3_tokens(i), None, name=f"c3_{i}.prm"))
+    mrs.append(_mr(tmp_path, scans.c1_tokens(), None, name="c1.prm"))
+    got = ctx.plan_batch_dev(mrs)
+    j = 0
+    for s, mr in enumerate(mrs):
+        for b in range(mr.ri.n_blocks):
+            cb.plan_block(mr, b)
+            g = got[j]
+            assert (g["scan"], g["block"]) == (s, b)
+            # the host struct keeps the pull start as size_t (a negative start
+            # wraps); the device plan keeps it as int32
+            v = int(mr.ri.idx_pull_start) & 0xffffffff
+            assert g["idx_pull_start"] == (v - (1 << 32) if v >= (1 << 31) else v)
+            assert g["n_proj_pull"] == int(mr.ri.n_proj_pull)
+            assert g["n_half_turns"] == cb.block_sizes(mr).n_half_turns
+            assert g["idx_block_slice_start"] == int(mr.ri.cb.idx_block_slice_start)
+            assert g["idx_block_slice_end"] == int(mr.ri.cb.idx_block_slice_end)
+            assert g["block_slice_start"] == np.float32(mr.ri.cb.block_slice_start)
+            assert g["block_slice_end"] == np.float32(mr.ri.cb.block_slice_end)
+            j += 1
+    assert j == got.size
+    print(f"device planner: {j} block plans of {len(mrs)} scans equal the host's")
+    for mr in mrs:
+        cb.free_metadata(mr)
