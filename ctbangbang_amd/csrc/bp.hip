@@ -497,41 +497,37 @@ constexpr auto bp_kernel_of() {
 
 // Persistent form: a grid of at most the resident workgroups works through
 // the (block, tile) items of many blocks (jobs).  Eight queues, one per XCD
-// (workgroup b starts on queue b % 8, the XCD the hardware dispatches it to):
-// queue q holds band q of every job, the contiguous tiles
-// [floor(q n / 8), floor((q + 1) n / 8)) of its n tiles, in job order, so
-// the tiles an XCD runs at one time lie side by side and share its L2.  A
-// workgroup takes items from its own queue, then from the others in turn,
-// and returns when all eight are empty (every workgroup reaches that).
+// (workgroup b runs on XCD b % 8, where the hardware dispatches it): queue q
+// holds band q of every job, the contiguous tiles
+// [floor(q n / 8), floor((q + 1) n / 8)) of its n tiles, in job order; a
+// launch covers the items of jobs [job_first, job_end).  The
+// g = gridDim / 8 workgroups of queue q take its items in a fixed stride
+// (workgroup b: items b / 8, b / 8 + g, ...), so the g tiles an XCD runs at
+// one time lie side by side and share its L2, as in one launch per block.
+// (Items handed out by an atomic counter instead were measured 18% slower
+// on C1.)  Every workgroup returns after its last item.
 template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
 __device__ __forceinline__ void bp_persistent_body(const BpQueues &Q, float2 *lds2) {
-    __shared__ int item_slot;
-    const int q0 = blockIdx.x & 7;
-    for (int qq = 0; qq < 8; qq++) {
-        const int q = (q0 + qq) & 7;
-        const int *pre = Q.band_pre + q * (Q.n_jobs + 1);
-        const int len = __builtin_amdgcn_readfirstlane(pre[Q.n_jobs]);
-        for (;;) {
-            if (threadIdx.x == 0) item_slot = atomicAdd(Q.counters + q, 1);
-            __syncthreads();
-            // (the same value in every lane: scalar from here on)
-            const int item = __builtin_amdgcn_readfirstlane(item_slot);
-            __syncthreads();                    // every thread has read the slot (and left the last tile)
-            if (item >= len) break;
-            // the job holding this item: the last j with pre[j] <= item
-            int lo = 0, hi = Q.n_jobs - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (__builtin_amdgcn_readfirstlane(pre[mid]) <= item) lo = mid;
-                else hi = mid - 1;
-            }
-            // the job through the scalar cache: the plan kernels wrote the
-            // jobs before this launch and nothing writes them during it
-            const __attribute__((address_space(4))) BpArgs &a =
-                *(const __attribute__((address_space(4))) BpArgs *)(Q.jobs + lo);
-            const int tile = (q * a.n_tiles) / 8 + (item - __builtin_amdgcn_readfirstlane(pre[lo]));
-            bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, 0, lds2);
+    const int q = blockIdx.x & 7;
+    const int per_q = gridDim.x >> 3;
+    const int *pre = Q.band_pre + q * (Q.n_jobs + 1);
+    const int len = __builtin_amdgcn_readfirstlane(pre[Q.job_end]);
+    for (int item = __builtin_amdgcn_readfirstlane(pre[Q.job_first]) + (blockIdx.x >> 3); item < len;
+         item += per_q) {
+        // the job holding this item: the last j with pre[j] <= item
+        int lo = Q.job_first, hi = Q.job_end - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (__builtin_amdgcn_readfirstlane(pre[mid]) <= item) lo = mid;
+            else hi = mid - 1;
         }
+        // the job through the scalar cache: the plan kernels wrote the jobs
+        // before this launch and nothing writes them during it
+        const __attribute__((address_space(4))) BpArgs &a =
+            *(const __attribute__((address_space(4))) BpArgs *)(Q.jobs + lo);
+        const int tile = (q * a.n_tiles) / 8 + (item - __builtin_amdgcn_readfirstlane(pre[lo]));
+        __syncthreads();                        // every thread has left the previous tile (LDS reuse)
+        bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, 0, lds2);
     }
 }
 
@@ -761,7 +757,9 @@ int launch_backproject(const BpArgs &a0, hipStream_t stream) {
 
 int launch_backproject_persistent(const BpArgs &sample, int variant, const BpQueues &Q, int total_items,
                                   int max_groups, hipStream_t stream) {
-    if (!Q.jobs || !Q.band_pre || !Q.counters || Q.n_jobs <= 0 || variant < 0) return CTBB_ERR_ARG;
+    if (!Q.jobs || !Q.band_pre || Q.n_jobs <= 0 || Q.job_first < 0 || Q.job_end <= Q.job_first ||
+        Q.job_end > Q.n_jobs || variant < 0)
+        return CTBB_ERR_ARG;
     if (total_items <= 0) return CTBB_OK;
     return with_variant(variant, [&](auto nz, auto zs, auto cpt, auto tpr) {
         return launch_p<decltype(nz)::value, decltype(cpt)::value, decltype(tpr)::value, decltype(zs)::value>(

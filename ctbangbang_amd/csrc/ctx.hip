@@ -1323,7 +1323,7 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
         if (!*e && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return CTBB_ERR_HIP;
     // host-built inputs of the device planner, one upload: per scan its
     // ScanPlanIn (in group order) and table positions; per group the plans,
-    // jobs, view-table offsets, queue prefixes and counters
+    // jobs, view-table offsets and queue prefixes
     size_t tpos_total = 0;
     std::vector<size_t> tpos_off(n_scans);
     for (int s = 0; s < n_scans; s++) {
@@ -1333,7 +1333,7 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
     if ((st = c->tpos_buf.ensure(tpos_total * sizeof(double)))) return st;
     auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
     struct GroupLayout {
-        size_t scans, plans, jobs, vt_off, band_pre, counters, vtab;
+        size_t scans, plans, jobs, vt_off, band_pre, vtab;
     };
     std::vector<GroupLayout> lay(groups.size());
     size_t bytes = 0, vtab_at = 0;
@@ -1349,8 +1349,6 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
         bytes = align(bytes + (nb + 1) * sizeof(size_t));
         lay[g].band_pre = bytes;
         bytes = align(bytes + 8 * (nb + 1) * sizeof(int));
-        lay[g].counters = bytes;
-        bytes = align(bytes + 8 * sizeof(int));
         lay[g].vtab = vtab_at;
         vtab_at += groups[g].views_total;
     }
@@ -1398,8 +1396,7 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
         std::memcpy(fin.data() + fin_off[s], blob.data(), blob.size());
     }
     if ((st = upload(c, c->fin_tab, fin.data(), fin.size()))) return st;
-    // the device work list of every group: plans, jobs, view tables, queues;
-    // all queue counters start at zero
+    // the device work list of every group: plans, jobs, view tables, queues
     for (size_t g = 0; g < groups.size(); g++) {
         st = launch_plan_batch(reinterpret_cast<const ScanPlanIn *>(dplan + lay[g].scans),
                                static_cast<int>(groups[g].scans.size()), groups[g].n_blocks, groups[g].views_max,
@@ -1408,104 +1405,110 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
                                reinterpret_cast<size_t *>(dplan + lay[g].vt_off),
                                reinterpret_cast<int *>(dplan + lay[g].band_pre), c->stream);
         if (st) return st;
-        st = fill_zero(reinterpret_cast<float *>(dplan + lay[g].counters), 8, c->stream);
-        if (st) return st;
     }
     // the backprojection stream starts after the work list
     if (hipEventRecord(c->ev_sync, c->stream) != hipSuccess || hipStreamWaitEvent(c->stream_bp, c->ev_sync, 0) != hipSuccess)
         return CTBB_ERR_HIP;
     const bool split = c->opt.proj_dtype != CTBB_PROJ_BF16 && c->opt.filter_mode == CTBB_FILTER_FFT32;
+    struct JobRef {
+        int s, b;
+        size_t slot;        // element offset of the block's filtered projections in the arena
+    };
     for (size_t g = 0; g < groups.size(); g++) {
         const int k = static_cast<int>(g & 1);
         // preparation into arena k once the group that used it before (g - 2)
         // is backprojected and finalized
         if (g >= 2 && hipStreamWaitEvent(c->stream, c->ev_gdone[k], 0) != hipSuccess) return CTBB_ERR_HIP;
-        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return CTBB_ERR_HIP;
-        c->capturing = true;
+        std::vector<JobRef> jobs;
         size_t slot_at = 0;
-        bool first = g == 0;
-        if (split) {
-            // FP32 projections, FFT filter: first the adaptive filter and the
-            // rebinning of every block, straight into its slot (low-register
-            // kernels that run beside the previous group's backprojection),
-            // then the filter of every slot in place (the FFT kernel needs
-            // the registers the backprojection holds)
-            for (int s : groups[g].scans) {
-                recon_metadata *m = mrs[s];
-                for (int b = 0; b < info[s].n_blocks && !st; b++) {
-                    ctbb::plan_block(m, b);
+        for (int s : groups[g].scans) {
+            for (int b = 0; b < info[s].n_blocks; b++) jobs.push_back({s, b, slot_at + b * info[s].slot_elems});
+            slot_at += info[s].slot_elems * info[s].n_blocks;
+        }
+        // Job ranges, each prepared (one captured graph) and then
+        // backprojected (one persistent launch).  The first group's
+        // preparation overlaps nothing, so it goes in ranges of 1, 2, 4, ...
+        // blocks, each prepared while the previous range is backprojected;
+        // later groups are one range, prepared during the previous group.
+        std::vector<std::pair<int, int>> ranges;
+        const int nj = static_cast<int>(jobs.size());
+        if (g == 0) {
+            for (int j0 = 0, len = 1; j0 < nj; j0 += len, len *= 2) ranges.push_back({j0, std::min(nj, j0 + len)});
+        } else {
+            ranges.push_back({0, nj});
+        }
+        const BatchScan &s0 = info[groups[g].scans.front()];
+        for (const auto &rg : ranges) {
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return CTBB_ERR_HIP;
+            c->capturing = true;
+            auto slot_ptr = [&](const JobRef &j) { return c->arena[k].as<char>() + j.slot * eb; };
+            if (split) {
+                // FP32 projections, FFT filter: first the adaptive filter and
+                // the rebinning of every block, straight into its slot
+                // (low-register kernels that run beside the backprojection
+                // before), then the filter of every slot in place (the FFT
+                // kernel needs the registers the backprojection holds)
+                for (int j = rg.first; j < rg.second && !st; j++) {
+                    recon_metadata *m = mrs[jobs[j].s];
+                    ctbb::plan_block(m, jobs[j].b);
                     const float *win = nullptr;
-                    st = block_window(c, m, d_raws[s], n_raw[s], &win);
+                    st = block_window(c, m, d_raws[jobs[j].s], n_raw[jobs[j].s], &win);
                     if (!st && c->opt.adaptive_filter) {
-                        st = do_adaptive(c, m, win, c->raw_af.as<float>(), first);
+                        st = do_adaptive(c, m, win, c->raw_af.as<float>(), g == 0 && j == 0);
                         win = c->raw_af.as<float>();
                     }
-                    float *slot = c->arena[k].as<float>() + slot_at + b * info[s].slot_elems;
-                    if (!st) st = do_rebin(c, m, win, slot, static_cast<int>(prep_views(m)));
-                    first = false;
+                    if (!st)
+                        st = do_rebin(c, m, win, reinterpret_cast<float *>(slot_ptr(jobs[j])),
+                                      static_cast<int>(prep_views(m)));
                 }
-                slot_at += info[s].slot_elems * info[s].n_blocks;
-                if (st) break;
-            }
-            slot_at = 0;
-            for (int s : groups[g].scans) {
-                recon_metadata *m = mrs[s];
-                for (int b = 0; b < info[s].n_blocks && !st; b++) {
-                    ctbb::plan_block(m, b);
-                    float *slot = c->arena[k].as<float>() + slot_at + b * info[s].slot_elems;
+                for (int j = rg.first; j < rg.second && !st; j++) {
+                    recon_metadata *m = mrs[jobs[j].s];
+                    ctbb::plan_block(m, jobs[j].b);
+                    float *slot = reinterpret_cast<float *>(slot_ptr(jobs[j]));
                     st = do_filter(c, c->taps.as<float>(), static_cast<int>(m->cg.n_channels_oversampled),
                                    prep_views(m) * m->cg.n_rows, slot, slot, 0);
                 }
-                slot_at += info[s].slot_elems * info[s].n_blocks;
-                if (st) break;
-            }
-        } else {
-            for (int s : groups[g].scans) {
-                recon_metadata *m = mrs[s];
-                for (int b = 0; b < info[s].n_blocks && !st; b++) {
-                    ctbb::plan_block(m, b);
+            } else {
+                for (int j = rg.first; j < rg.second && !st; j++) {
+                    recon_metadata *m = mrs[jobs[j].s];
+                    ctbb::plan_block(m, jobs[j].b);
                     const float *win = nullptr;
-                    st = block_window(c, m, d_raws[s], n_raw[s], &win);
-                    if (!st)
-                        st = prep_block(c, m, win, c->arena[k].as<char>() + (slot_at + b * info[s].slot_elems) * eb,
-                                        first);
-                    first = false;
+                    st = block_window(c, m, d_raws[jobs[j].s], n_raw[jobs[j].s], &win);
+                    if (!st) st = prep_block(c, m, win, slot_ptr(jobs[j]), g == 0 && j == 0);
                 }
-                slot_at += info[s].slot_elems * info[s].n_blocks;
-                if (st) break;
             }
+            c->capturing = false;
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+            if (st || ec != hipSuccess || !graph) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return st ? st : CTBB_ERR_HIP;
+            }
+            hipGraphExec_t exec = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess || !exec) return CTBB_ERR_HIP;
+            c->live_graphs.push_back(exec);
+            EventPair e = c->begin("prepare", c->stream);
+            if (hipGraphLaunch(exec, c->stream) != hipSuccess) return CTBB_ERR_HIP;
+            c->end(e);
+            if (hipEventRecord(c->ev_gprep, c->stream) != hipSuccess ||
+                hipStreamWaitEvent(c->stream_bp, c->ev_gprep, 0) != hipSuccess)
+                return CTBB_ERR_HIP;
+            // one persistent backprojection over every (block, tile) of the range
+            BpQueues Q{};
+            Q.jobs = reinterpret_cast<const BpArgs *>(dplan + lay[g].jobs);
+            Q.band_pre = reinterpret_cast<const int *>(dplan + lay[g].band_pre);
+            Q.n_jobs = groups[g].n_blocks;
+            Q.job_first = rg.first;
+            Q.job_end = rg.second;
+            int total = 0;
+            for (int j = rg.first; j < rg.second; j++) total += info[jobs[j].s].tmpl.n_tiles;
+            EventPair eb2 = c->begin("backproject", c->stream_bp);
+            st = launch_backproject_persistent(s0.tmpl, s0.variant, Q, total, 0, c->stream_bp);
+            c->end(eb2);
+            if (st) return st;
         }
-        c->capturing = false;
-        hipGraph_t graph = nullptr;
-        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-        if (st || ec != hipSuccess || !graph) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return st ? st : CTBB_ERR_HIP;
-        }
-        hipGraphExec_t exec = nullptr;
-        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess || !exec) return CTBB_ERR_HIP;
-        c->live_graphs.push_back(exec);
-        EventPair e = c->begin("prepare", c->stream);
-        if (hipGraphLaunch(exec, c->stream) != hipSuccess) return CTBB_ERR_HIP;
-        c->end(e);
-        if (hipEventRecord(c->ev_gprep, c->stream) != hipSuccess ||
-            hipStreamWaitEvent(c->stream_bp, c->ev_gprep, 0) != hipSuccess)
-            return CTBB_ERR_HIP;
-        // one persistent backprojection over every (block, tile) of the group
-        BpQueues Q{};
-        Q.jobs = reinterpret_cast<const BpArgs *>(dplan + lay[g].jobs);
-        Q.band_pre = reinterpret_cast<const int *>(dplan + lay[g].band_pre);
-        Q.counters = reinterpret_cast<int *>(dplan + lay[g].counters);
-        Q.n_jobs = groups[g].n_blocks;
-        const BatchScan &s0 = info[groups[g].scans.front()];
-        int total = 0;
-        for (int s : groups[g].scans) total += info[s].n_blocks * info[s].tmpl.n_tiles;
-        EventPair eb2 = c->begin("backproject", c->stream_bp);
-        st = launch_backproject_persistent(s0.tmpl, s0.variant, Q, total, 0, c->stream_bp);
-        c->end(eb2);
-        if (st) return st;
         // finalize every scan of the group behind it
         size_t img_at = 0;
         for (int s : groups[g].scans) {

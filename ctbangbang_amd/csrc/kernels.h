@@ -39,8 +39,8 @@ int launch_backproject(const BpArgs &a, hipStream_t stream);
 struct BpQueues {
     const BpArgs *jobs;        // [n_jobs] (device memory)
     const int *band_pre;       // [8][n_jobs + 1]: items of queue q before job j (band q of each job)
-    int *counters;             // [8], zero at launch
     int n_jobs;
+    int job_first, job_end;    // the launch covers jobs [job_first, job_end)
 };
 // Fills the staging shape (tpr, tpr_log2) of a and returns an identifier of
 // its kernel variant (equal identifiers: one persistent launch may hold
