@@ -1426,14 +1426,18 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
             slot_at += info[s].slot_elems * info[s].n_blocks;
         }
         // Job ranges, each prepared (one captured graph) and then
-        // backprojected (one persistent launch).  The first group's
-        // preparation overlaps nothing, so it goes in ranges of 1, 2, 4, ...
-        // blocks, each prepared while the previous range is backprojected;
-        // later groups are one range, prepared during the previous group.
+        // backprojected (one persistent launch).  A group is one range,
+        // prepared while the previous group is backprojected, except the
+        // first group: its preparation has nothing to overlap, so it runs
+        // one block per range, each block prepared beside the previous
+        // block's backprojection (beside the backprojection the adaptive
+        // filter and the rebin of a block take about 0.9 of the block's
+        // backprojection time, C1: 5.4 of 6 ms, so larger ranges would not
+        // be ready in time).
         std::vector<std::pair<int, int>> ranges;
         const int nj = static_cast<int>(jobs.size());
         if (g == 0) {
-            for (int j0 = 0, len = 1; j0 < nj; j0 += len, len *= 2) ranges.push_back({j0, std::min(nj, j0 + len)});
+            for (int j0 = 0; j0 < nj; j0++) ranges.push_back({j0, j0 + 1});
         } else {
             ranges.push_back({0, nj});
         }
@@ -1608,7 +1612,12 @@ int ctbb_reconstruct_batch_dev(ctbb_ctx *c, int n_scans, struct recon_metadata *
             return CTBB_ERR_ARG;
     }
     if (hipSetDevice(c->device) != hipSuccess) return CTBB_ERR_HIP;
-    if (c->opt.batch_mode == CTBB_BATCH_PIPELINE) return batch_pipeline(c, n_scans, mrs, d_raws, n_raw_readings, d_finals);
+    // The persistent form gains by running the blocks of many scans per
+    // launch; a batch of one scan runs the per-block pipeline (C1: 48.7 ms
+    // against 51.1 ms persistent, which adds the device planning and the
+    // per-block graphs and launches of its first group).
+    if (c->opt.batch_mode == CTBB_BATCH_PIPELINE || n_scans == 1)
+        return batch_pipeline(c, n_scans, mrs, d_raws, n_raw_readings, d_finals);
     int st = ensure_taps(c, mrs[0]);
     if (st) return st;
     st = ensure_maps(c, mrs[0]);

@@ -140,7 +140,8 @@ typedef enum {
 
 typedef enum {
     CTBB_BATCH_PERSISTENT = 0,  /* groups of scans: captured preparation graph + one persistent
-                                   backprojection launch per group (default) */
+                                   backprojection launch per group (default; a batch of one
+                                   scan runs the pipeline form) */
     CTBB_BATCH_PIPELINE = 1     /* one launch per block, preparation beside the previous block */
 } ctbb_batch_mode;
 
