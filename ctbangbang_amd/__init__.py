@@ -187,6 +187,15 @@ def plan_block(mr, block_idx):
     _check(lib().ctbb_plan_block(C.byref(mr), int(block_idx)), "plan_block")
 
 
+def pull_start(mr):
+    """First reading of the planned block's pull as a signed number.  The
+    reference keeps it in a size_t (recon_structs.h), so a pull that starts
+    before reading 0 (possible with TableDir in near the start of the data)
+    wraps; the device paths zero-pad those readings."""
+    v = int(mr.ri.idx_pull_start) & ((1 << 64) - 1)
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
 def block_sizes(mr):
     s = Sizes()
     _check(lib().ctbb_block_sizes(C.byref(mr), C.byref(s)), "block_sizes")

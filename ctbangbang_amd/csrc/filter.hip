@@ -474,9 +474,20 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(FilterArgs a) {
 int launch_filter(const FilterArgs &a, hipStream_t stream) {
     if (!a.in || !a.out || !a.taps || a.n <= 0) return CTBB_ERR_ARG;
     if (a.rows == 0) return CTBB_OK;
-    if (a.mode == CTBB_FILTER_FFT32 && 2 * a.n <= FFT_L) {
-        // each workgroup reads its two rows completely before writing them,
-        // so in == out is allowed here; grid.x is limited to 2^31 - 1
+    const bool fft = a.mode == CTBB_FILTER_FFT32 && 2 * a.n <= FFT_L;
+    {
+        // In place (out == in) only for the FFT kernel with FP32 output: each
+        // of its workgroups reads its two rows completely before writing the
+        // same two rows.  Narrower (bfloat16) output rows, an offset overlap,
+        // or the tiled kernels (which read rows other workgroups write) would
+        // race, so any other overlap is refused.
+        const uintptr_t ib = reinterpret_cast<uintptr_t>(a.in), ob = reinterpret_cast<uintptr_t>(a.out);
+        const uintptr_t ie = ib + a.rows * static_cast<size_t>(a.n) * sizeof(float);
+        const uintptr_t oe = ob + a.rows * static_cast<size_t>(a.n) * (a.out_bf16 ? sizeof(uint16_t) : sizeof(float));
+        if (ib < oe && ob < ie && !(fft && !a.out_bf16 && ib == ob)) return CTBB_ERR_ARG;
+    }
+    if (fft) {
+        // grid.x is limited to 2^31 - 1
         if (!a.spec) return CTBB_ERR_ARG;
         const size_t n_pairs = (a.rows + 1) / 2;
         const size_t max_pairs = 0x7fffffffu;
@@ -493,7 +504,6 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
         return CTBB_OK;
     }
     // FFT32 with 2n > 4096 uses the FP32 Toeplitz product below
-    if (a.in == a.out && (a.out_bf16 || a.n % 64 == 0)) return CTBB_ERR_ARG;  // tiled kernel reads rows after writing others
     if (a.n % 64 != 0) {
         if (a.rows > 0x7fffffffu) return CTBB_ERR_SHAPE;
         size_t lds = 3 * static_cast<size_t>(a.n) * sizeof(float);

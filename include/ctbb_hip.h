@@ -236,9 +236,11 @@ int ctbb_rebin_filter_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
 int ctbb_rebin_dev(ctbb_ctx *ctx, const struct recon_metadata *mr,
                    const float *d_raw, float *d_rebin);
 
-/* Ramp filter rows of length n_channels_oversampled in place:
- * rows[r][l] <- sum_k taps[n + l - k] * rows[r][k]. d_out may alias d_in only
- * for CTBB_PROJ_F32 output. */
+/* Ramp filter rows of length n_channels_oversampled:
+ * rows[r][l] <- sum_k taps[n + l - k] * rows[r][k].  d_out == d_in (in place)
+ * is allowed only for the CTBB_FILTER_FFT32 filter (rows of at most 2048
+ * channels) with CTBB_PROJ_F32 output; any other overlap of d_in and d_out
+ * returns CTBB_ERR_ARG. */
 int ctbb_filter_rows_dev(ctbb_ctx *ctx, const float *d_taps, int n, size_t n_rows_total,
                          const float *d_in, void *d_out);
 

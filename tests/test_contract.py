@@ -275,8 +275,29 @@ def test_driver_empty_prm(tmp_path):
     assert "TableFeed:" in toks and "TableDirInt:" in toks and len(toks) == 30
 
 
-def test_binary_reader_roundtrip(tmp_path, mini_dir):
-    tok = scans.mini_tokens("none")
+@pytest.mark.parametrize("table_dir,edge", [("out", False), ("in", True)])
+def test_binary_reader_roundtrip(tmp_path, mini_dir, table_dir, edge):
+    """The binary reader returns the pull's frames from the file, zeros for
+    readings outside it.  With TableDir in and the first slice at the lowest
+    position the planner accepts, the first pull starts before reading 0
+    (a wrapped size_t in the reference's struct): those frames are zeros.
+    (With the table moving in, that is the highest start the planner
+    accepts.)"""
+    tok = scans.mini_tokens("none", table_dir=table_dir)
+    if edge:
+        for k in range(199, -1, -1):
+            tok["StartPos"] = 0.5 * k
+            tok["EndPos"] = tok["StartPos"] + (tok["NSlices"] - 1) * 2.4
+            q = str(tmp_path / "probe.prm")
+            cb.write_prm(q, tok)
+            try:
+                m = cb.metadata_from_prm(q, install_dir=mini_dir)
+            except cb.CTBBError:
+                continue
+            cb.plan_block(m, 0)
+            assert cb.pull_start(m) < 0
+            cb.free_metadata(m)
+            break
     tok["RawDataDir"] = str(tmp_path)
     tok["RawOffset"] = 64
     p = str(tmp_path / "m.prm")
@@ -292,10 +313,11 @@ def test_binary_reader_roundtrip(tmp_path, mini_dir):
     assert cb.lib().ctbb_extract_projections(C.byref(mr)) == 0
     n = mr.ri.n_proj_pull * frame
     got = np.ctypeslib.as_array(mr.ctd.raw, shape=(n,))
-    s = mr.ri.idx_pull_start
+    s = cb.pull_start(mr)
+    assert (s < 0) == edge
     want = np.zeros(n, dtype=np.float32)
-    hi = min(tok["Readings"], s + mr.ri.n_proj_pull)
-    want[:(hi - s) * frame] = data[s * frame:hi * frame]
+    lo, hi = max(s, 0), min(tok["Readings"], s + mr.ri.n_proj_pull)
+    want[(lo - s) * frame:(hi - s) * frame] = data[lo * frame:hi * frame]
     assert np.array_equal(got, want)
     cb.free_metadata(mr)
 

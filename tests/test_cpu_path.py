@@ -116,7 +116,7 @@ def test_rebin_filter_cpu_vs_oracle(tmp_path, mini_dir, ffs):
     tok = scans.mini_tokens(ffs, nrows=8 if ffs in ("z", "both") else 4)
     mr = _mr(tmp_path, tok, mini_dir)
     cb.plan_block(mr, 0)
-    raw = cb.synth_scan(mr, scans.phantom_for(tok, 3), int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull))
+    raw = cb.synth_scan(mr, scans.phantom_for(tok, 3), cb.pull_start(mr), int(mr.ri.n_proj_pull))
     got = _block_rebin(mr, raw)
     want = oracle.filter_fft(oracle.rebin(mr, raw, double=True), cb.load_filter(mr))
     err = np.abs(got - want).max() / np.abs(want).max()
@@ -141,7 +141,7 @@ def test_cpu_path_end_to_end_hu(tmp_path, mini_dir):
     nblk = mr.ri.n_slices_block * mr.rp.nx * mr.rp.ny
     for b in range(mr.ri.n_blocks):
         cb.plan_block(mr, b)
-        s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+        s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
         win = np.zeros((n, raw_all.shape[1]), dtype=np.float32)
         lo, hi = max(s0, 0), min(s0 + n, n_read)
         win[lo - s0:hi - s0] = raw_all[lo:hi]

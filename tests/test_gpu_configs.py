@@ -93,7 +93,7 @@ def oracle_volume(mr, raw_h, cols, adaptive):
     blocks = []
     for b in range(mr.ri.n_blocks):
         cb.plan_block(mr, b)
-        win = _window(raw_h, int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull), frame)
+        win = _window(raw_h, cb.pull_start(mr), int(mr.ri.n_proj_pull), frame)
         if adaptive == "kk":
             win, _ = oracle.adaptive_filter(mr, win)
         elif adaptive is not None:
@@ -206,7 +206,7 @@ def test_c1_adaptive_filter_bit_exact(ctx, c1_case):
     import oracle
     mr = c1_case["mr"]
     cb.plan_block(mr, 3)
-    s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+    s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
     frame = c1_case["frame"]
     win = c1_case["raw"][s0 * frame:(s0 + n) * frame]
     out = torch.empty_like(win)
@@ -228,7 +228,7 @@ def test_c1_rebin_filter_block(ctx, c1_case):
     mr = c1_case["mr"]
     cb.plan_block(mr, 3)
     s = cb.block_sizes(mr)
-    s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+    s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
     frame = c1_case["frame"]
     win = c1_case["raw"][s0 * frame:(s0 + n) * frame]
     ctx.set_options(filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
@@ -252,7 +252,7 @@ def test_c1_adaptive_fixed_threshold_bit_exact(ctx, c1_case):
     import oracle
     mr = c1_case["mr"]
     cb.plan_block(mr, 2)
-    s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+    s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
     frame = c1_case["frame"]
     T = float(np.float32(np.log(1e5 / 3e4)))
     ctx.set_options(adaptive_threshold=T)

@@ -39,7 +39,7 @@ def _mr(tmp_path, tokens, install, name="s.prm"):
 def _raw_pull(ctx, mr, pp):
     s = cb.block_sizes(mr)
     d = torch.empty(s.raw_pull, dtype=torch.float32, device="cuda")
-    ctx.synth_scan_dev(mr, pp, mr.ri.idx_pull_start, mr.ri.n_proj_pull, d)
+    ctx.synth_scan_dev(mr, pp, cb.pull_start(mr), mr.ri.n_proj_pull, d)
     ctx.synchronize()
     return d
 
@@ -90,8 +90,22 @@ def test_ramp_filter_fft_in_place(ctx):
     taps_d = dev(taps)
     ctx.filter_rows_dev(taps_d, n, rows, a, out)
     ctx.filter_rows_dev(taps_d, n, rows, a, a)
-    ctx.set_options(filter_mode=cb.FILTER_F32)
     assert np.array_equal(host(a).ravel(), host(out))
+    # any other overlap is refused before a kernel runs: bfloat16 output in
+    # place (its rows are half as wide), an offset overlap, and the tiled
+    # forms in place
+    # (one row of slack, so even the offset destination stays inside the buffer)
+    buf = dev(np.concatenate([x.ravel(), np.zeros(n, np.float32)]))
+    b = buf[:rows * n]
+    for mode, proj, dst in ((cb.FILTER_FFT32, cb.PROJ_BF16, b), (cb.FILTER_FFT32, cb.PROJ_F32, buf[n:]),
+                            (cb.FILTER_F32, cb.PROJ_F32, b), (cb.FILTER_BF16X3, cb.PROJ_F32, b)):
+        ctx.set_options(filter_mode=mode, proj_dtype=proj)
+        with pytest.raises(cb.CTBBError) as e:
+            ctx.filter_rows_dev(taps_d, n, rows, b, dst)
+        assert e.value.code == 1                                # CTBB_ERR_ARG
+    ctx.synchronize()
+    assert np.array_equal(host(b).reshape(rows, n), x)        # untouched
+    ctx.set_options(filter_mode=cb.FILTER_F32, proj_dtype=cb.PROJ_F32)
 
 
 @pytest.mark.parametrize("mode", [cb.FILTER_F32, cb.FILTER_BF16X3, cb.FILTER_FFT32])
@@ -354,7 +368,7 @@ def _oracle_volume(mr, raw_all, taps, adaptive, mu):
     blocks = []
     for b in range(mr.ri.n_blocks):
         cb.plan_block(mr, b)
-        s0, n = mr.ri.idx_pull_start, mr.ri.n_proj_pull
+        s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
         win = np.zeros((n, frame), dtype=np.float32)
         lo, hi = max(s0, 0), min(s0 + n, n_read)
         win[lo - s0:hi - s0] = raw_all[lo:hi]
@@ -389,6 +403,48 @@ def test_end_to_end_hu(ctx, tmp_path, mini_dir, ffs, table_dir):
     cb.free_metadata(mr)
 
 
+@pytest.mark.parametrize("ffs", ["none", "both"])
+def test_end_to_end_pull_before_data(ctx, tmp_path, mini_dir, ffs):
+    """TableDir in with the first slice at the highest position the planner
+    accepts: the first block's pull starts before reading 0 (the reference
+    keeps the start in a size_t, so it wraps), and the readings before the
+    data are zeros.  Device-resident and host-streaming reconstructions
+    agree bit for bit and match the FP64 oracle fed the same zero-padded
+    window."""
+    base = scans.mini_tokens(ffs, nrows=8 if ffs == "both" else 4, nx=32, table_dir="in")
+    mr = None
+    for k in range(199, -1, -1):
+        tok = dict(base)
+        tok["StartPos"] = 0.5 * k
+        tok["EndPos"] = tok["StartPos"] + (tok["NSlices"] - 1) * 2.4
+        try:
+            mr = _mr(tmp_path, tok, mini_dir, name=f"edge_{k}.prm")
+        except cb.CTBBError:
+            continue
+        break
+    assert mr is not None
+    cb.plan_block(mr, 0)
+    assert cb.pull_start(mr) < 0
+    n_read = tok["Readings"]
+    frame = mr.cg.n_rows_raw * mr.cg.n_channels
+    raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda")
+    ctx.synth_scan_dev(mr, scans.phantom_for(tok, 6), 0, n_read, raw)
+    s = cb.block_sizes(mr)
+    out = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
+    ctx.reconstruct_dev(mr, raw, n_read, out)
+    got = host(out).reshape(-1, mr.rp.nx, mr.rp.ny)
+    h_out = torch.empty(s.final_stack, dtype=torch.float32).pin_memory()
+    ctx.reconstruct_host(mr, raw.cpu().pin_memory(), n_read, h_out)
+    assert np.array_equal(h_out.numpy().reshape(got.shape), got)
+    want = _oracle_volume(mr, host(raw).reshape(n_read, frame), cb.load_filter(mr), True, 0.02)
+    d = got.astype(np.float64) - want
+    rms, mx = float(np.sqrt(np.mean(d ** 2))), float(np.abs(d).max())
+    print(f"pull before data {ffs}: start {tok['StartPos']}, first pull {cb.pull_start(mr)}; HU rms {rms:.4f} max {mx:.4f}")
+    assert rms <= 1.0 and mx <= 10.0
+    cb.free_metadata(mr)
+
+
 @pytest.mark.parametrize("ffs", ["none", "z", "phi", "both"])
 def test_pipeline_matches_stage_chain(ctx, tmp_path, mini_dir, ffs):
     """The whole-scan pipeline (block b + 1 prepared on one stream while block
@@ -413,7 +469,7 @@ def test_pipeline_matches_stage_chain(ctx, tmp_path, mini_dir, ffs):
     for b in range(n_blocks):
         cb.plan_block(mr, b)
         sb = cb.block_sizes(mr)
-        s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+        s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
         assert s0 >= 0 and s0 + n <= n_read
         win = raw[s0 * frame:(s0 + n) * frame]
         af = torch.empty_like(win)
@@ -448,7 +504,7 @@ def test_reconstruct_host_matches_dev(ctx, tmp_path, mini_dir, ffs):
     raw = torch.empty(n_read * frame, dtype=torch.float32, device="cuda")
     ctx.synth_scan_dev(mr, scans.phantom_for(tok, 5), 0, n_read, raw)
     cb.plan_block(mr, int(mr.ri.n_blocks) - 1)
-    n_use = int(mr.ri.idx_pull_start) + int(mr.ri.n_proj_pull) - 100   # last pull runs past the data
+    n_use = cb.pull_start(mr) + int(mr.ri.n_proj_pull) - 100   # last pull runs past the data
     s = cb.block_sizes(mr)
     ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
     out_d = torch.empty(s.final_stack, dtype=torch.float32, device="cuda")
@@ -682,10 +738,10 @@ def test_autotune_backprojection(ctx, tmp_path, mini_dir):
                     bp_tile=0)
     cache = str(tmp_path / "tune.txt")
     cb.plan_block(mr, 0)
-    before = (int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull), int(mr.ri.cb.block_idx))
+    before = (cb.pull_start(mr), int(mr.ri.n_proj_pull), int(mr.ri.cb.block_idx))
     t1 = ctx.autotune_backprojection(mr, cache)
     assert t1 in (16, 32) and ctx.options().bp_tile == t1
-    assert (int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull), int(mr.ri.cb.block_idx)) == before
+    assert (cb.pull_start(mr), int(mr.ri.n_proj_pull), int(mr.ri.cb.block_idx)) == before
     text = open(cache).read()
     assert text.count("\n") == 1 and text.split()[1] == str(t1)
     ctx.set_options(bp_tile=0)
@@ -742,7 +798,7 @@ def test_nonfinite_raw_is_reported(ctx, tmp_path, mini_dir):
     ctx.synchronize()
     want = host(out)
     cb.plan_block(mr, 0)
-    mid = int(mr.ri.idx_pull_start) + int(mr.ri.n_proj_pull) // 2
+    mid = cb.pull_start(mr) + int(mr.ri.n_proj_pull) // 2
     for bad in (float("nan"), float("inf")):
         r = raw.clone()
         r[mid * frame + 3 * mr.cg.n_channels + 60] = bad
@@ -817,7 +873,7 @@ def test_end_to_end_fixed_threshold(ctx, tmp_path, mini_dir):
     blocks, smoothed = [], 0
     for b in range(mr.ri.n_blocks):
         cb.plan_block(mr, b)
-        s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+        s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
         win = np.zeros((n, frame), dtype=np.float32)
         lo, hi = max(s0, 0), min(s0 + n, n_read)
         win[lo - s0:hi - s0] = raw_h[lo:hi]
@@ -852,7 +908,7 @@ def test_device_planner_matches_host(ctx, tmp_path):
             assert (g["scan"], g["block"]) == (s, b)
             # the host struct keeps the pull start as size_t (a negative start
             # wraps); the device plan keeps it as int32
-            v = int(mr.ri.idx_pull_start) & 0xffffffff
+            v = cb.pull_start(mr) & 0xffffffff
             assert g["idx_pull_start"] == (v - (1 << 32) if v >= (1 << 31) else v)
             assert g["n_proj_pull"] == int(mr.ri.n_proj_pull)
             assert g["n_half_turns"] == cb.block_sizes(mr).n_half_turns

@@ -31,7 +31,7 @@ def _oracle_stack(mr, raw_all, conventions):
     blocks = []
     for b in range(mr.ri.n_blocks):
         cb.plan_block(mr, b)
-        s0, n = int(mr.ri.idx_pull_start), int(mr.ri.n_proj_pull)
+        s0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
         win = np.zeros((n,) + raw_all.shape[1:], dtype=np.float32)
         lo, hi = max(s0, 0), min(s0 + n, n_read)
         win[lo - s0:hi - s0] = raw_all[lo:hi]

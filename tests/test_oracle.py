@@ -189,7 +189,7 @@ def _cylinder_raw(mr, x0, y0, R, mu, ffs):
     dr = cg.src_to_det * mr.rp.coll_slicewidth / (4.0 * (cg.src_to_det - cg.r_f) * np.tan(cg.anode_angle))
     table = {"none": [(0, 0, 0)], "phi": [(da, 0, 0), (-da, 0, 0)], "z": [(0, -dr, 0), (0, dr, 1)],
              "both": [(da, -dr, 0), (-da, -dr, 0), (da, dr, 1), (-da, dr, 1)]}[ffs]
-    n0, n = mr.ri.idx_pull_start, mr.ri.n_proj_pull
+    n0, n = cb.pull_start(mr), int(mr.ri.n_proj_pull)
     idx = np.arange(n0, n0 + n)
     alpha = np.deg2rad(mr.rp.tube_start_angle + 360.0 * idx / cg.n_proj_ffs)
     ch = np.arange(cg.n_channels)
@@ -230,7 +230,7 @@ def test_rebin_analytic_cylinder(tmp_path, ffs):
     out = oracle.rebin(mr, raw, double=True)
     cg = mr.cg
     v = np.arange(out.shape[0])
-    theta = np.deg2rad(mr.rp.tube_start_angle + 360.0 * mr.ri.idx_pull_start / cg.n_proj_ffs) + \
+    theta = np.deg2rad(mr.rp.tube_start_angle + 360.0 * cb.pull_start(mr) / cg.n_proj_ffs) + \
         2 * np.pi * (v + cg.add_projections) / cg.n_proj_turn
     c = np.arange(cg.n_channels_oversampled)
     s = cg.r_f * (c - 2 * cg.central_channel) * cg.fan_angle_increment / 2
@@ -255,7 +255,7 @@ def test_rebin_constant_and_precision(tmp_path, mini_dir, ffs):
     out = oracle.rebin(mr, np.full(n, 3.25, np.float32))
     assert np.allclose(out, 3.25, rtol=1e-6)
     pp = scans.phantom_for(scans.mini_tokens(ffs), 2)
-    raw = cb.synth_scan(mr, pp, mr.ri.idx_pull_start, mr.ri.n_proj_pull)
+    raw = cb.synth_scan(mr, pp, cb.pull_start(mr), mr.ri.n_proj_pull)
     a = oracle.rebin(mr, raw, double=True)
     b = oracle.rebin(mr, raw, double=False)
     assert np.abs(a - b).max() <= 1e-4 * np.abs(raw).max()
@@ -298,10 +298,10 @@ def test_adaptive_threshold_property(tmp_path, mini_dir, eccentric):
     tok = scans.mini_tokens("both", nrows=8)
     mr = _mr(tmp_path, tok, mini_dir)
     cb.plan_block(mr, 0)
-    raw = cb.synth_scan(mr, scans.phantom_for(tok, 3), mr.ri.idx_pull_start, mr.ri.n_proj_pull)
+    raw = cb.synth_scan(mr, scans.phantom_for(tok, 3), cb.pull_start(mr), mr.ri.n_proj_pull)
     if eccentric:
         # per-view maxima oscillating 1:5 with the tube angle -> eccentricity 1
-        a = 4 * np.pi * (mr.ri.idx_pull_start + np.arange(mr.ri.n_proj_pull)) / mr.cg.n_proj_ffs
+        a = 4 * np.pi * (cb.pull_start(mr) + np.arange(mr.ri.n_proj_pull)) / mr.cg.n_proj_ffs
         rng = np.random.default_rng(0)
         raw = (raw * (3 + 2 * np.cos(a))[:, None, None] *
                (1 + 0.05 * rng.standard_normal(raw.shape[0]))[:, None, None]).astype(np.float32)
