@@ -98,6 +98,21 @@ struct ctbb_ctx {
     hipStream_t stream_bp = nullptr;
     DevBuf filt[kPipeDepth], image_stack2;
     hipEvent_t ev_filt[kPipeDepth] = {}, ev_bp[kPipeDepth] = {}, ev_sync = nullptr;
+    // Head of a whole-volume call: the first block's adaptive filter and
+    // rebin run on `stream_head`, into workspaces of their own, after
+    // ev_prep_done (recorded on `stream` after the previous call's last
+    // preparation), so in back-to-back calls they run beside the previous
+    // call's last backprojection; ev_head hands their result to the ramp
+    // filter on `stream`.
+    hipStream_t stream_head = nullptr;
+    hipEvent_t ev_prep_done = nullptr, ev_head = nullptr;
+    // the head's own workspaces (adaptive-filter output and work, rebinned
+    // rows, zero-padded pull), so it touches nothing the previous call or
+    // other entry points use
+    DevBuf head_af, head_work, head_rebin, head_pad;
+    // finalize tap lists last uploaded to fin_tab (unchanged lists are not
+    // uploaded again: the upload waits for the compute stream)
+    std::vector<char> fin_blob;
     // cached host-side keys
     std::string taps_key, maps_key;
     int map_kind = 0, map_sets = 0;
@@ -294,7 +309,8 @@ int ensure_maps(ctbb_ctx *ctx, const recon_metadata *mr) {
 
 // n_views_max < 0: every rebinned view of the block; otherwise at most the
 // first n_views_max views.
-int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, int n_views_max = -1) {
+int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, int n_views_max = -1,
+             hipStream_t on = nullptr) {
     int st = ensure_maps(ctx, mr);
     if (st) return st;
     const ct_geom &cg = mr->cg;
@@ -316,8 +332,9 @@ int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float 
     if (n_views_max >= 0 && n_views_max < a.n_proj_out) a.n_proj_out = n_views_max;
     a.reverse_rows = cg.reverse_row_interleave;
     a.nonfinite = ctx->nonfinite.as<unsigned>();
-    EventPair e = ctx->begin("rebin");
-    st = launch_rebin(a, ctx->stream);
+    hipStream_t s = on ? on : ctx->stream;
+    EventPair e = ctx->begin("rebin", s);
+    st = launch_rebin(a, s);
     ctx->end(e);
     return st;
 }
@@ -480,14 +497,18 @@ int do_finalize(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_image, f
     int mt = 0, nf = 0;
     int st = finalize_blob(mr, blob, &mt, &nf);
     if (st) return st;
-    st = upload(ctx, ctx->fin_tab, blob.data(), blob.size());
-    if (st) return st;
+    if (blob != ctx->fin_blob || !ctx->fin_tab.p) {
+        st = upload(ctx, ctx->fin_tab, blob.data(), blob.size());
+        if (st) return st;
+        ctx->fin_blob = blob;
+    }
     return launch_finalize_blob(ctx, mr, d_image, d_final, ctx->fin_tab.as<char>(), mt, nf, ctx->stream);
 }
 
 // vec4: the 16-byte apply kernel (for a GPU that runs nothing else; the
 // one-sample kernel needs few registers and runs beside a backprojection).
-int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, bool vec4 = true) {
+int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, bool vec4 = true,
+                hipStream_t on = nullptr, DevBuf *work = nullptr) {
     const ct_geom &cg = mr->cg;
     AdaptiveArgs a{};
     a.raw = d_raw;
@@ -499,11 +520,13 @@ int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, flo
     a.strength = mr->rp.adaptive_filtration_s;
     a.vec4 = vec4 ? 1 : 0;
     a.fixed_threshold = ctx->opt.adaptive_threshold;
-    int st = ctx->af_work.ensure(6 * static_cast<size_t>(a.n_proj) * sizeof(float));
+    DevBuf &wb = work ? *work : ctx->af_work;
+    int st = wb.ensure(6 * static_cast<size_t>(a.n_proj) * sizeof(float));
     if (st) return st;
-    a.work = ctx->af_work.as<float>();
-    EventPair e = ctx->begin("adaptive");
-    st = launch_adaptive(a, ctx->stream);
+    a.work = wb.as<float>();
+    hipStream_t s = on ? on : ctx->stream;
+    EventPair e = ctx->begin("adaptive", s);
+    st = launch_adaptive(a, s);
     ctx->end(e);
     return st;
 }
@@ -540,21 +563,27 @@ int backproject_dev(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin
 // Adaptive filter -> rebin -> ramp filter of the current block (mr->ri.cb)
 // on ctx->stream, raw pull at `win`, filtered projections into d_out.
 // first: the first item of a pipeline (no backprojection is running yet).
-int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out, bool first) {
+// head: run the adaptive filter and the rebin on this stream (after
+// ev_prep_done; the caller has made the zero-padded pull, if any, there
+// too) and hand over to ctx->stream for the filter.
+int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out, bool first,
+               hipStream_t head = nullptr) {
     const ct_geom &cg = mr->cg;
     const size_t frame = cg.n_rows_raw * cg.n_channels;
     const size_t n = mr->ri.n_proj_pull;
     ctbb_sizes sz;
     ctbb_block_sizes(mr, &sz);
     int st;
+    DevBuf &af = head ? c->head_af : c->raw_af;
+    DevBuf &rb = head ? c->head_rebin : c->rebin_tmp;
     if (c->opt.adaptive_filter) {
-        st = c->raw_af.ensure(n * frame * sizeof(float));
+        st = af.ensure(n * frame * sizeof(float));
         if (st) return st;
-        st = do_adaptive(c, mr, win, c->raw_af.as<float>(), first);
+        st = do_adaptive(c, mr, win, af.as<float>(), first, head, head ? &c->head_work : nullptr);
         if (st) return st;
-        win = c->raw_af.as<float>();
+        win = af.as<float>();
     }
-    st = c->rebin_tmp.ensure(sz.rebin * sizeof(float));
+    st = rb.ensure(sz.rebin * sizeof(float));
     if (st) return st;
     // The backprojection reads views i + k * half (i < half, k < n_ht) only
     // (view table, reference src/backproject.cu:39); the pull's remaining
@@ -571,10 +600,13 @@ int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out, b
         views += 1;
         rows += 1;
     }
-    st = do_rebin(c, mr, win, c->rebin_tmp.as<float>(), static_cast<int>(views));
+    st = do_rebin(c, mr, win, rb.as<float>(), static_cast<int>(views), head);
     if (st) return st;
-    return do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), rows,
-                     c->rebin_tmp.as<float>(), d_out, c->opt.proj_dtype == CTBB_PROJ_BF16);
+    if (head && (hipEventRecord(c->ev_head, head) != hipSuccess ||
+                 hipStreamWaitEvent(c->stream, c->ev_head, 0) != hipSuccess))
+        return CTBB_ERR_HIP;
+    return do_filter(c, c->taps.as<float>(), static_cast<int>(cg.n_channels_oversampled), rows, rb.as<float>(), d_out,
+                     c->opt.proj_dtype == CTBB_PROJ_BF16);
 }
 
 int ensure_pipeline(ctbb_ctx *c, size_t rebin_bytes) {
@@ -585,6 +617,14 @@ int ensure_pipeline(ctbb_ctx *c, size_t rebin_bytes) {
                 hipEventCreateWithFlags(&c->ev_bp[i], hipEventDisableTiming) != hipSuccess)
                 return CTBB_ERR_HIP;
         if (hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming) != hipSuccess) return CTBB_ERR_HIP;
+    }
+    if (!c->stream_head) {
+        // ev_prep_done starts out recorded after everything queued so far
+        if (hipStreamCreateWithFlags(&c->stream_head, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_prep_done, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_head, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(c->ev_prep_done, c->stream) != hipSuccess)
+            return CTBB_ERR_HIP;
     }
     for (DevBuf &b : c->filt) {
         const int st = b.ensure(rebin_bytes);
@@ -601,13 +641,15 @@ int ensure_pipeline(ctbb_ctx *c, size_t rebin_bytes) {
 // filtered-projection buffer item % kPipeDepth (after the backprojection of
 // item - kPipeDepth has read it), then backproject block b into d_image on
 // stream_bp once the preparation is done.
+// head_alone: the head runs on an otherwise idle context (no earlier
+// backprojection to share the GPU with).
 int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int item, int b, const float *win, const BpView *d_vtab,
-                   float *d_image) {
+                   float *d_image, hipStream_t head = nullptr, bool head_alone = false) {
     const int k = item % ctbb_ctx::kPipeDepth;
     DevBuf &out = c->filt[k];
     if (item >= ctbb_ctx::kPipeDepth && hipStreamWaitEvent(c->stream, c->ev_bp[k], 0) != hipSuccess)
         return CTBB_ERR_HIP;
-    int st = prep_block(c, mr, win, out.p, item == 0);
+    int st = prep_block(c, mr, win, out.p, item == 0 && (!head || head_alone), head);
     if (st) return st;
     if (hipEventRecord(c->ev_filt[k], c->stream) != hipSuccess ||
         hipStreamWaitEvent(c->stream_bp, c->ev_filt[k], 0) != hipSuccess)
@@ -680,9 +722,10 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     if (c->stream_bp) (void)hipStreamSynchronize(c->stream_bp);
+    if (c->stream_head) (void)hipStreamSynchronize(c->stream_head);
     for (DevBuf *b : {&c->raw_in, &c->raw_af, &c->rebin_tmp, &c->image_block, &c->image_stack, &c->af_work,
                       &c->vtab, &c->maps, &c->taps, &c->spec, &c->fin_tab, &c->pull[0], &c->pull[1],
-                      &c->image_stack2, &c->nonfinite})
+                      &c->image_stack2, &c->nonfinite, &c->head_af, &c->head_work, &c->head_rebin, &c->head_pad})
         b->release();
     for (DevBuf &b : c->filt) b.release();
     for (int i = 0; i < 2; i++) {
@@ -696,6 +739,9 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     }
     if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
     if (c->stream_bp) (void)hipStreamDestroy(c->stream_bp);
+    for (hipEvent_t e : {c->ev_prep_done, c->ev_head})
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream_head) (void)hipStreamDestroy(c->stream_head);
     for (hipGraphExec_t g : c->live_graphs) (void)hipGraphExecDestroy(g);
     c->live_graphs.clear();
     for (DevBuf *b : {&c->arena[0], &c->arena[1], &c->img_arena[0], &c->img_arena[1], &c->plan_buf, &c->tpos_buf})
@@ -735,6 +781,7 @@ void *ctbb_ctx_stream(ctbb_ctx *c) { return c ? static_cast<void *>(c->stream) :
 int ctbb_ctx_synchronize(ctbb_ctx *c) {
     if (!c) return CTBB_ERR_ARG;
     if (c->stream_bp && hipStreamSynchronize(c->stream_bp) != hipSuccess) return CTBB_ERR_HIP;
+    if (c->stream_head && hipStreamSynchronize(c->stream_head) != hipSuccess) return CTBB_ERR_HIP;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return CTBB_ERR_HIP;
     for (hipGraphExec_t g : c->live_graphs) (void)hipGraphExecDestroy(g);
     c->live_graphs.clear();
@@ -745,6 +792,7 @@ int ctbb_ctx_kernel_time(ctbb_ctx *c, const char *family, float *total_ms, int *
     if (!c || !family || !total_ms || !launches) return CTBB_ERR_ARG;
     // (the streams only: a pending non-finite report stays for synchronize)
     if ((c->stream_bp && hipStreamSynchronize(c->stream_bp) != hipSuccess) ||
+        (c->stream_head && hipStreamSynchronize(c->stream_head) != hipSuccess) ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return CTBB_ERR_HIP;
     *total_ms = 0.0f;
@@ -974,28 +1022,46 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
         ctbb_block_sizes(mr, &sz);
         const long start = static_cast<long>(static_cast<int>(mr->ri.idx_pull_start));
         const long n = static_cast<long>(mr->ri.n_proj_pull);
+        // the first block's adaptive filter and rebin on the head stream
+        // (beside a previous call's last backprojection)
+        hipStream_t head = nullptr;
+        bool head_alone = false;
+        if (b == 0) {
+            head = c->stream_head;
+            // nothing of the context still queued: the head does not share
+            // the GPU with a backprojection (kernels for an idle GPU)
+            head_alone = hipStreamQuery(c->stream) == hipSuccess && hipStreamQuery(c->stream_bp) == hipSuccess &&
+                         hipStreamQuery(head) == hipSuccess;
+            if (hipStreamWaitEvent(head, c->ev_prep_done, 0) != hipSuccess) return CTBB_ERR_HIP;
+        }
+        hipStream_t ps = head ? head : c->stream;
         const float *win;
         if (start >= 0 && start + n <= static_cast<long>(n_raw_readings)) {
             win = d_raw + static_cast<size_t>(start) * frame;
         } else {
             // pull reaches outside the scan: zero-padded copy
-            st = c->raw_in.ensure(static_cast<size_t>(n) * frame * sizeof(float));
+            DevBuf &pad = head ? c->head_pad : c->raw_in;
+            st = pad.ensure(static_cast<size_t>(n) * frame * sizeof(float));
             if (st) return st;
-            float *dst = c->raw_in.as<float>();
-            st = fill_zero(dst, static_cast<size_t>(n) * frame, c->stream);
+            float *dst = pad.as<float>();
+            st = fill_zero(dst, static_cast<size_t>(n) * frame, ps);
             if (st) return st;
             long lo = start < 0 ? 0 : start, hi = start + n;
             if (hi > static_cast<long>(n_raw_readings)) hi = static_cast<long>(n_raw_readings);
             if (hi > lo &&
                 hipMemcpyAsync(dst + static_cast<size_t>(lo - start) * frame, d_raw + static_cast<size_t>(lo) * frame,
                                static_cast<size_t>(hi - lo) * frame * sizeof(float), hipMemcpyDeviceToDevice,
-                               c->stream) != hipSuccess)
+                               ps) != hipSuccess)
                 return CTBB_ERR_HIP;
             win = dst;
         }
-        st = pipeline_block(c, mr, b, b, win, c->vtab.as<BpView>() + off[b], d_image);
+        st = pipeline_block(c, mr, b, b, win, c->vtab.as<BpView>() + off[b], d_image, head, head_alone);
         if (st) return st;
     }
+    // the next call's head starts once every preparation of this call is done
+    // (so it runs beside this call's last backprojection, not beside a
+    // preparation) and the head's workspaces have been read
+    if (hipEventRecord(c->ev_prep_done, c->stream) != hipSuccess) return CTBB_ERR_HIP;
     st = join_pipeline(c);
     if (st) return st;
     return do_finalize(c, mr, d_image, d_final);

@@ -445,6 +445,39 @@ def test_end_to_end_pull_before_data(ctx, tmp_path, mini_dir, ffs):
     cb.free_metadata(mr)
 
 
+def test_back_to_back_calls(ctx, tmp_path, mini_dir):
+    """Whole-volume calls queued back to back without synchronizing (the next
+    call's first preparation runs beside the previous call's last
+    backprojection, in workspaces of its own): every output equals the one
+    of the same call made alone, also when the scans (and so the finalize
+    tables, view tables and buffer sizes) change from call to call."""
+    items = []
+    for i, (ffs, nx, rot) in enumerate((("both", 32, 24), ("none", 48, 20), ("both", 32, 24))):
+        tok = scans.mini_tokens(ffs, nrows=8, nx=nx, rotations=rot, n_slices=30)
+        mr = _mr(tmp_path, tok, mini_dir, name=f"b2b_{i}.prm")
+        frame = mr.cg.n_rows_raw * mr.cg.n_channels
+        raw = torch.empty(tok["Readings"] * frame, dtype=torch.float32, device="cuda")
+        ctx.synth_scan_dev(mr, scans.phantom_for(tok, 7 + i), 0, tok["Readings"], raw)
+        items.append((mr, raw, tok["Readings"]))
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
+    alone = []
+    for mr, raw, n in items:
+        out = torch.empty(cb.block_sizes(mr).final_stack, dtype=torch.float32, device="cuda")
+        ctx.reconstruct_dev(mr, raw, n, out)
+        ctx.synchronize()
+        alone.append(host(out).copy())
+    outs = [torch.zeros(cb.block_sizes(mr).final_stack, dtype=torch.float32, device="cuda") for mr, _, _ in items]
+    torch.cuda.synchronize()
+    for _ in range(2):
+        for (mr, raw, n), out in zip(items, outs):
+            ctx.reconstruct_dev(mr, raw, n, out)
+    ctx.synchronize()
+    for a, out in zip(alone, outs):
+        assert np.array_equal(host(out), a)
+    for mr, _, _ in items:
+        cb.free_metadata(mr)
+
+
 @pytest.mark.parametrize("ffs", ["none", "z", "phi", "both"])
 def test_pipeline_matches_stage_chain(ctx, tmp_path, mini_dir, ffs):
     """The whole-scan pipeline (block b + 1 prepared on one stream while block
