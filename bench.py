@@ -407,7 +407,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(blocks shard with no exchange across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(the figure is one GPU's throughput; blocks would shard across GPUs with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
     if args.stage == "copy":
         stage_copy(args)

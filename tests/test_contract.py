@@ -134,7 +134,7 @@ def _plan(tmp_path, tokens, name="x.prm", install=None):
 ])
 def test_planning_known_answers(tmp_path, cfg, want):
     """Shapes of the reference's planning arithmetic (src/setup.cu:291-297,
-    483-679) for the standard scans, hand-computed in the survey."""
+    483-679) for the standard scans, hand-computed from that arithmetic."""
     mr = _plan(tmp_path, getattr(scans, cfg + "_tokens")())
     cg, ri = mr.cg, mr.ri
     assert cg.n_proj_ffs == want["n_proj_ffs"]

@@ -22,7 +22,7 @@ Bars (HU = 1000 (mu - mu_w) / mu_w, mu_w = 0.02 /mm):
   plain FP32 forms     RMS <= 0.05 HU, max <= 2 HU (the max allows the rare
                        |q| = 1 steps of the weighting, where one view's
                        sample enters in full or not at all)
-  precision forms      RMS <= 1 HU, max <= 10 HU
+  precision forms      RMS <= 1 HU (the precision forms' bar), max <= 10 HU
   adaptive filter      bit-exact against the FP32 restatement
   rebin + ramp filter  max |HIP - FP64| <= 1e-4 of the block's max |value|
 """
@@ -189,7 +189,7 @@ def test_c1_end_to_end(ctx, c1_case):
                                        (cb.FILTER_FFT32, cb.PROJ_BF16), (cb.FILTER_F32, cb.PROJ_F32)])
 def test_c2_precision_forms(ctx, c1_case, mode, proj):
     """Precision forms at C1 size: HU error per dtype variant against the
-    FP64 oracle (RMS <= 1 HU)."""
+    FP64 oracle (precision forms' bar: RMS <= 1 HU)."""
     got = _recon(ctx, c1_case, filter_mode=mode, proj_dtype=proj)
     names = {cb.FILTER_F32: "f32", cb.FILTER_BF16X3: "bf16x3", cb.FILTER_F16X3: "f16x3", cb.FILTER_FFT32: "fft32"}
     label = f"c2 filter {names[mode]}, projections {'bf16' if proj == cb.PROJ_BF16 else 'f32'}"

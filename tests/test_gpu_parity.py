@@ -600,7 +600,7 @@ def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir, mode, group_m
                                        (cb.FILTER_FFT32, cb.PROJ_BF16)])
 def test_end_to_end_precision_form_hu(ctx, tmp_path, mini_dir, mode, proj):
     """Precision form (split MFMA filter, optionally BF16 projections with FP32
-    accumulation) vs the FP64 oracle: the north-star bar is <= 1 HU RMS."""
+    accumulation) vs the FP64 oracle: the precision forms' bar is <= 1 HU RMS."""
     tok = scans.mini_tokens("both", nrows=8, nx=32)
     mr = _mr(tmp_path, tok, mini_dir)
     pp = scans.phantom_for(tok, 5)
