@@ -83,6 +83,7 @@ struct ctbb_ctx {
     ctbb::BpView *noalloc_vtab = nullptr;   // this device's `scratch` array
     DevBuf raw_in, raw_af, rebin_tmp, image_block, image_stack, af_work;
     DevBuf vtab, maps, taps, spec, fin_tab;
+    int spec_len = 0;          // transform length of `spec` (3072 or 4096; 0: none)
     // 1 once a rebin met a non-finite output sample (read and cleared by
     // ctbb_ctx_synchronize)
     DevBuf nonfinite;
@@ -249,15 +250,42 @@ void dft_pow2(std::vector<std::complex<double>> &x) {
     }
 }
 
-// FFT32 form of the filter: H = DFT_4096(taps zero-padded) / 4096 and the
-// twiddles e^(-2 pi i t / 4096), evaluated in FP64 and rounded once, into
-// ctx->spec (only when 2n <= 4096; wider filters use the FP32 product).
+// DFT of length L (forward sign) of the first `count` values of x zero-padded,
+// FP64, directly from a table of the L roots of unity.
+std::vector<std::complex<double>> dft_direct(const float *x, size_t count, size_t L) {
+    std::vector<std::complex<double>> w(L), X(L);
+    for (size_t t = 0; t < L; t++) {
+        const double a = -2.0 * M_PI * static_cast<double>(t) / static_cast<double>(L);
+        w[t] = std::complex<double>(std::cos(a), std::sin(a));
+    }
+    for (size_t k = 0; k < L; k++) {
+        std::complex<double> acc = 0.0;
+        size_t t = 0;                                   // m k mod L
+        for (size_t m = 0; m < count; m++) {
+            acc += static_cast<double>(x[m]) * w[t];
+            t += k;
+            if (t >= L) t -= L;
+        }
+        X[k] = acc;
+    }
+    return X;
+}
+
+// FFT32 form of the filter: H = DFT_L(taps zero-padded) / L and the
+// twiddles e^(-2 pi i t / L), evaluated in FP64 and rounded once, into
+// ctx->spec, with L = 3072 for 2n <= 3072 and 4096 for 2n <= 4096 (wider
+// filters use the FP32 product).
 int make_spec(ctbb_ctx *ctx, const float *taps, size_t count) {
-    constexpr size_t L = 4096;
-    if (count > L) return CTBB_OK;
-    std::vector<std::complex<double>> h(L, 0.0);
-    for (size_t m = 0; m < count; m++) h[m] = taps[m];
-    dft_pow2(h);
+    if (count > 4096) return CTBB_OK;
+    const size_t L = count <= 3072 ? 3072 : 4096;
+    std::vector<std::complex<double>> h;
+    if (L == 4096) {
+        h.assign(L, 0.0);
+        for (size_t m = 0; m < count; m++) h[m] = taps[m];
+        dft_pow2(h);
+    } else {
+        h = dft_direct(taps, count, L);
+    }
     std::vector<float> sp(4 * L);
     for (size_t k = 0; k < L; k++) {
         sp[2 * k] = static_cast<float>(h[k].real() / L);
@@ -266,7 +294,11 @@ int make_spec(ctbb_ctx *ctx, const float *taps, size_t count) {
         sp[2 * L + 2 * k] = static_cast<float>(std::cos(a));
         sp[2 * L + 2 * k + 1] = static_cast<float>(std::sin(a));
     }
-    return upload(ctx, ctx->spec, sp.data(), sp.size() * sizeof(float));
+    ctx->spec_len = 0;
+    const int st = upload(ctx, ctx->spec, sp.data(), sp.size() * sizeof(float));
+    if (st) return st;
+    ctx->spec_len = static_cast<int>(L);
+    return CTBB_OK;
 }
 
 int ensure_taps(ctbb_ctx *ctx, const recon_metadata *mr) {
@@ -345,6 +377,7 @@ int do_filter(ctbb_ctx *ctx, const float *d_taps, int n, size_t rows, const floa
     f.out = d_out;
     f.taps = d_taps;
     f.spec = ctx->spec.p ? ctx->spec.as<float2>() : nullptr;
+    f.fft_len = ctx->spec.p ? ctx->spec_len : 0;
     f.rows = rows;
     f.n = n;
     f.mode = ctx->opt.filter_mode;

@@ -316,12 +316,19 @@ __global__ __launch_bounds__(256) void conv_valu_kernel(FilterArgs a) {
 // (n + l - k lies in [1, 2n - 1]).  Two rows a, b travel as one complex
 // signal a + i b: h is real, so y = (a (*) h) + i (b (*) h).
 // The host supplies H = DFT_L(h) / L and the twiddles e^(-2 pi i t / L),
-// both evaluated in FP64 and rounded once.
-// Transform: Stockham radix-16 (three passes of 256 threads x 16 points,
-// the passes in between through LDS with one pad entry per 16 to spread
-// the banks); the inverse is conj(DFT(conj(Y))).
+// both evaluated in FP64 and rounded once.  L = 3072 where 2n <= 3072 (the
+// shipped scanners: 1472 and 1344 oversampled channels), else 4096.
+// Transforms (Stockham, the passes in between through LDS with one pad
+// entry per 16 to spread the banks; the inverse is conj(DFT(conj(Y)))):
+//   L = 4096: radix 16, 16, 16, each pass 256 threads x 16 points;
+//   L = 3072: forward radix 12, 16, 16 (256 threads x 12 points, then
+//   192 x 16), the inverse in the reverse order 16, 16, 12, so each
+//   transform starts from the layout the previous one ended in: 25% fewer
+//   VALU instructions than L = 4096 for the same rows.
 constexpr int FFT_L = 4096;
 constexpr int FFT_LDS = FFT_L + FFT_L / 16;
+constexpr int FFT_L3 = 3072;
+constexpr int FFT_LDS3 = FFT_L3 + FFT_L3 / 16;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
@@ -367,12 +374,50 @@ __device__ __forceinline__ void fft16(float2 (&v)[16]) {
     t = v[11]; v[11] = v[14]; v[14] = t;
 }
 
+// in-place 3-point DFT, forward sign: X1 = a0 + w a1 + w^2 a2, w = e^(-2 pi i / 3)
+__device__ __forceinline__ void dft3(float2 &a0, float2 &a1, float2 &a2) {
+    constexpr float S3 = 0.86602540378443865f;   // sin(pi / 3)
+    const float2 t = cadd(a1, a2);
+    const float2 d = make_float2(S3 * (a1.x - a2.x), S3 * (a1.y - a2.y));
+    const float2 c = make_float2(fmaf(-0.5f, t.x, a0.x), fmaf(-0.5f, t.y, a0.y));
+    a0 = cadd(a0, t);
+    a1 = make_float2(c.x + d.y, c.y - d.x);     // c - i d
+    a2 = make_float2(c.x - d.y, c.y + d.x);     // c + i d
+}
+
+// 12-point DFT of v[0 .. 11] in natural order, as 3 x 4: inner 3-point DFTs
+// over r2 (r = r1 + 4 r2), twiddles W12^(r1 k2), outer 4-point DFTs over r1;
+// X[k2 + 3 k1] ends in slot k1 + 4 k2.
+__device__ __forceinline__ void fft12(float2 (&v)[16]) {
+    constexpr float S3 = 0.86602540378443865f;
+#pragma unroll
+    for (int r1 = 0; r1 < 4; r1++) dft3(v[r1], v[r1 + 4], v[r1 + 8]);
+    // W12^m = (cos(pi m / 6), -sin(pi m / 6))
+    v[5] = cmul(v[5], make_float2(S3, -0.5f));     // r1 1, k2 1: m 1
+    v[9] = cmul(v[9], make_float2(0.5f, -S3));     // r1 1, k2 2: m 2
+    v[6] = cmul(v[6], make_float2(0.5f, -S3));     // r1 2, k2 1: m 2
+    v[10] = cmul(v[10], make_float2(-0.5f, -S3));  // r1 2, k2 2: m 4
+    v[7] = make_float2(v[7].y, -v[7].x);           // r1 3, k2 1: m 3 (-i)
+    v[11] = make_float2(-v[11].x, -v[11].y);       // r1 3, k2 2: m 6 (-1)
+#pragma unroll
+    for (int k2 = 0; k2 < 3; k2++) dft4(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3]);
+    // natural order: X[k] sits in slot k / 3 + 4 (k % 3)
+    float2 o[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) o[k] = v[k / 3 + 4 * (k % 3)];
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = o[k];
+}
+
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
 
 // v[r] *= w^r for r = 1 .. 15, from b[m] = w^(2^m), m = 0..3: the other
 // powers are products of at most three of them (each product rounds once,
 // so the twiddles stay within a few FP32 ulps of the table values).
+// (R = 12: r = 1 .. 11 only)
+template <int R = 16>
 __device__ __forceinline__ void twiddle16(float2 (&v)[16], const float2 (&b)[4]) {
+    static_assert(R == 12 || R == 16, "radix 12 or 16");
     const float2 w3 = cmul(b[1], b[0]);
     const float2 w5 = cmul(b[2], b[0]);
     const float2 w6 = cmul(b[2], b[1]);
@@ -388,6 +433,7 @@ __device__ __forceinline__ void twiddle16(float2 (&v)[16], const float2 (&b)[4])
     v[9] = cmul(v[9], cmul(b[3], b[0]));
     v[10] = cmul(v[10], cmul(b[3], b[1]));
     v[11] = cmul(v[11], cmul(b[3], w3));
+    if (R == 12) return;
     v[12] = cmul(v[12], cmul(b[3], b[2]));
     v[13] = cmul(v[13], cmul(b[3], w5));
     v[14] = cmul(v[14], cmul(b[3], w6));
@@ -423,6 +469,127 @@ __device__ __forceinline__ void fft4096(float2 (&v)[16], float2 *sh, int j, cons
     for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 256 * r)];
     twiddle16(v, b);
     fft16(v);                                           // pass Ns = 256
+}
+
+// Forward 3072-point DFT, radix 12, 16, 16.  On entry (every thread)
+// v[r] = x[j + 256 r], r < 12; on exit (threads j < 192) v[r] = X[j + 192 r],
+// r < 16.  tw: e^(-2 pi i t / 3072).  Stockham: the pass of radix R and
+// stride Ns takes butterfly j from x[j + r N / R], turns it by
+// w^r, w = e^(-2 pi i (j mod Ns) / (Ns R)), and leaves output r at
+// (j / Ns) Ns R + j mod Ns + r Ns.
+__device__ __forceinline__ void fft3072(float2 (&v)[16], float2 *sh, int j, const float2 *__restrict__ tw) {
+    const bool act = j < 192;               // butterflies of the radix-16 passes
+    float2 b[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) b[m] = tw[act ? (16 * (j % 12)) << m : 0];
+    fft12(v);                                           // R 12, Ns 1
+    __syncthreads();                                    // earlier readers of sh are done
+#pragma unroll
+    for (int r = 0; r < 12; r++) sh[fpad(12 * j + r)] = v[r];
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 192 * r)];
+        twiddle16(v, b);
+#pragma unroll
+        for (int m = 0; m < 4; m++) b[m] = tw[j << m];
+        fft16(v);                                       // R 16, Ns 12
+    }
+    __syncthreads();
+    if (act) {
+        const int base = (j / 12) * 192 + j % 12;
+#pragma unroll
+        for (int r = 0; r < 16; r++) sh[fpad(base + 12 * r)] = v[r];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 192 * r)];
+        twiddle16(v, b);
+        fft16(v);                                       // R 16, Ns 192
+    }
+}
+
+// The same DFT with the radices in reverse order (16, 16, 12): on entry
+// (threads j < 192) v[r] = x[j + 192 r], r < 16; on exit (every thread)
+// v[r] = X[j + 256 r], r < 12.
+__device__ __forceinline__ void fft3072_rev(float2 (&v)[16], float2 *sh, int j, const float2 *__restrict__ tw) {
+    const bool act = j < 192;
+    float2 b[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) b[m] = tw[act ? (12 * (j & 15)) << m : 0];
+    if (act) fft16(v);                                  // R 16, Ns 1
+    __syncthreads();                                    // earlier readers of sh are done
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) sh[fpad(16 * j + r)] = v[r];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = sh[fpad(j + 192 * r)];
+        twiddle16(v, b);
+        fft16(v);                                       // R 16, Ns 16
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) b[m] = tw[j << m];
+    __syncthreads();
+    if (act) {
+        const int base = (j >> 4) * 256 + (j & 15);
+#pragma unroll
+        for (int r = 0; r < 16; r++) sh[fpad(base + 16 * r)] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 12; r++) v[r] = sh[fpad(j + 256 * r)];
+    twiddle16<12>(v, b);
+    fft12(v);                                           // R 12, Ns 256
+}
+
+// L = 3072 form of fft_filter_kernel (2n <= 3072); spec = [H (3072)]
+// followed by [twiddles (3072)].
+__global__ __launch_bounds__(256) void fft3072_filter_kernel(FilterArgs a) {
+    __shared__ float2 sh[FFT_LDS3];
+    const int j = threadIdx.x;
+    const int n = a.n;
+    const float2 *__restrict__ H = a.spec;
+    const float2 *__restrict__ tw = a.spec + FFT_L3;
+    const size_t ra = 2 * static_cast<size_t>(blockIdx.x);
+    const bool has_b = ra + 1 < a.rows;
+    const float *xa = a.in + ra * n;
+    const float *xb = xa + n;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 12; r++) {
+        const int m = j + 256 * r;
+        v[r] = make_float2(m < n ? xa[m] : 0.0f, (m < n && has_b) ? xb[m] : 0.0f);
+    }
+    fft3072(v, sh, j, tw);
+    if (j < 192) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const float2 y = cmul(v[r], H[j + 192 * r]);
+            v[r] = make_float2(y.x, -y.y);
+        }
+    }
+    fft3072_rev(v, sh, j, tw);
+    // y = conj(v); keep y[n .. 2n)
+#pragma unroll
+    for (int r = 0; r < 12; r++) {
+        const int m = j + 256 * r;
+        if (m >= n && m < 2 * n) {
+            const size_t o = ra * n + (m - n);
+            if (a.out_bf16) {
+                uint16_t *ob = reinterpret_cast<uint16_t *>(a.out);
+                ob[o] = f32_to_bf16_rne(v[r].x);
+                if (has_b) ob[o + n] = f32_to_bf16_rne(-v[r].y);
+            } else {
+                float *of = reinterpret_cast<float *>(a.out);
+                of[o] = v[r].x;
+                if (has_b) of[o + n] = -v[r].y;
+            }
+        }
+    }
 }
 
 // One workgroup per pair of rows; spec = [H (L)] followed by [twiddles (L)]
@@ -474,7 +641,10 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(FilterArgs a) {
 int launch_filter(const FilterArgs &a, hipStream_t stream) {
     if (!a.in || !a.out || !a.taps || a.n <= 0) return CTBB_ERR_ARG;
     if (a.rows == 0) return CTBB_OK;
+    // (the spectrum's length a.fft_len: 3072 or 4096, 0 without one)
     const bool fft = a.mode == CTBB_FILTER_FFT32 && 2 * a.n <= FFT_L;
+    const bool fft3 = fft && a.fft_len == FFT_L3 && 2 * a.n <= FFT_L3;
+    if (fft && !fft3 && a.spec && a.fft_len != FFT_L) return CTBB_ERR_ARG;
     {
         // In place (out == in) only for the FFT kernel with FP32 output: each
         // of its workgroups reads its two rows completely before writing the
@@ -498,7 +668,10 @@ int launch_filter(const FilterArgs &a, hipStream_t stream) {
             if (a.out_bf16) s.out = reinterpret_cast<uint16_t *>(a.out) + 2 * p0 * a.n;
             else s.out = reinterpret_cast<float *>(a.out) + 2 * p0 * a.n;
             s.rows = a.rows - 2 * p0 < 2 * np ? a.rows - 2 * p0 : 2 * np;
-            hipLaunchKernelGGL(fft_filter_kernel, dim3(static_cast<unsigned>(np)), dim3(256), 0, stream, s);
+            if (fft3)
+                hipLaunchKernelGGL(fft3072_filter_kernel, dim3(static_cast<unsigned>(np)), dim3(256), 0, stream, s);
+            else
+                hipLaunchKernelGGL(fft_filter_kernel, dim3(static_cast<unsigned>(np)), dim3(256), 0, stream, s);
             if (hipGetLastError() != hipSuccess) return CTBB_ERR_HIP;
         }
         return CTBB_OK;

@@ -72,7 +72,8 @@ struct FilterArgs {
     const float *in;           // [rows][n]
     void *out;                 // [rows][n] float32 or bfloat16
     const float *taps;         // [2n]
-    const float2 *spec;        // FFT32: [H (4096)][twiddles (4096)]
+    const float2 *spec;        // FFT32: [H (fft_len)][twiddles (fft_len)]
+    int fft_len;               // length of spec's transform: 3072 (2n <= 3072) or 4096
     size_t rows;
     int n;
     int mode;                  // ctbb_filter_mode
