@@ -1065,6 +1065,10 @@ int ctbb_reconstruct_dev(ctbb_ctx *c, struct recon_metadata *mr, const float *d_
             // the GPU with a backprojection (kernels for an idle GPU)
             head_alone = hipStreamQuery(c->stream) == hipSuccess && hipStreamQuery(c->stream_bp) == hipSuccess &&
                          hipStreamQuery(head) == hipSuccess;
+            // a busy stream answers hipErrorNotReady, which is no failure:
+            // keep it out of the launch checks that read hipGetLastError
+            // (every earlier launch has been checked already)
+            (void)hipGetLastError();
             if (hipStreamWaitEvent(head, c->ev_prep_done, 0) != hipSuccess) return CTBB_ERR_HIP;
         }
         hipStream_t ps = head ? head : c->stream;
