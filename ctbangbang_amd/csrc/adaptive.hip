@@ -9,9 +9,10 @@
 // restatement:
 //   1. view maximum         one workgroup per view, tree max (order-free)
 //   2. 18-degree smoothing  one thread per view, sequential fmaf sum
-//   3. 180-degree min/max + eccentricity   one thread per view
-//   4. threshold bisection  one wavefront per view, exact integer counts over
-//      the view's window copied once to LDS
+//   3. 180-degree min/max + eccentricity, and
+//   4. threshold bisection  one wavefront per view: the view's window is
+//      copied once to LDS (min and max taken on the way, order-free), then
+//      exact integer counts over it
 //   5. 5x5x5 triangle smoothing of samples above threshold: one thread per
 //      sample (coalesced along channels)
 // A fixed-threshold mode (no counterpart in the reference) skips steps 1-4
@@ -72,32 +73,6 @@ __global__ void af_smooth(const float *sup, float *sm, int n_proj, int n_avg) {
     sm[p] = acc;
 }
 
-// reference preprocessing.cuh:115-163; one wavefront per view (min and max
-// are order-free)
-__global__ __launch_bounds__(256) void af_ecc(const float *sm, float *mx, float *mn, float *ecc, int n_proj,
-                                              int n_range) {
-    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (p >= n_proj) return;
-    const int off = n_range / 2;
-    const int max_idx = n_proj - 1;
-    float hi = 0.0f, lo = 10000.0f;
-    for (int i = threadIdx.x & 63; i < n_range; i += 64) {
-        int idx = p - off + i;
-        idx = idx < 0 ? 0 : (idx > max_idx ? max_idx : idx);
-        const float v = sm[idx];
-        hi = fmaxf(hi, v);
-        lo = fminf(lo, v);
-    }
-    hi = wave_max(hi);
-    lo = wave_min(lo);
-    if ((threadIdx.x & 63) != 0) return;
-    mx[p] = hi;
-    mn[p] = lo;
-    float e = 1 - lo / hi;
-    e = (e - 0.3f) / (0.5f - 0.3f);
-    ecc[p] = e > 1 ? 1.0f : (e < 0 ? 0.0f : e);
-}
-
 // The counting window of reference preprocessing.cuh:32-62 (clipped to the
 // pull), fixed for a view while the bisection moves T.
 __device__ __forceinline__ void af_window(int &start_idx, int &numel_range, int max_idx) {
@@ -119,13 +94,15 @@ __device__ __forceinline__ float af_rhs_wave(const float *win, int numel, float 
     return inv_pi_npf_a * static_cast<float>(cnt) * inv_npf;
 }
 
-// reference preprocessing.cuh:167-206; one wavefront per view.  The view's
-// counting window is copied to LDS once (kLds, when it fits in the dynamic
-// LDS the host gives the launch) and every bisection step counts from there;
-// otherwise each step reads it from memory.
+// reference preprocessing.cuh:115-206; one wavefront per view.  The view's
+// window is copied to LDS once (kLds, when it fits in the dynamic LDS the
+// host gives the launch), its minimum and maximum taken on the way (the
+// reference's 180-degree range, preprocessing.cuh:115-163: the same views as
+// the counting window, its clamped edge indices repeat values inside it),
+// and every bisection step counts from there; otherwise each step reads the
+// window from memory.
 template <bool kLds>
-__global__ __launch_bounds__(64) void af_thresh(const float *sm, const float *mx, const float *mn, const float *ecc,
-                                                float *thr, int n_proj, int n_range, int n_proj_ffs,
+__global__ __launch_bounds__(64) void af_thresh(const float *sm, float *thr, int n_proj, int n_range, int n_proj_ffs,
                                                 float strength) {
     extern __shared__ float win_lds[];
     const int p = blockIdx.x;
@@ -133,16 +110,27 @@ __global__ __launch_bounds__(64) void af_thresh(const float *sm, const float *mx
     int start = p - n_range / 2, numel = n_range;
     af_window(start, numel, max_idx);
     const float *win = sm + start;
+    float hi = 0.0f, lo = 10000.0f;     // the reference's starting values
+    for (int i = threadIdx.x; i < numel; i += 64) {
+        const float v = win[i];
+        hi = fmaxf(hi, v);
+        lo = fminf(lo, v);
+        if (kLds) win_lds[i] = v;
+    }
+    hi = wave_max(hi);
+    lo = wave_min(lo);
     if (kLds) {
-        for (int i = threadIdx.x; i < numel; i += 64) win_lds[i] = win[i];
         __syncthreads();
         win = win_lds;
     }
+    float e = 1 - lo / hi;
+    e = (e - 0.3f) / (0.5f - 0.3f);
+    const float ecc = e > 1 ? 1.0f : (e < 0 ? 0.0f : e);
     const float inv_pi = 1.0f / 3.1415368979f;
     const float inv_npf = 1.0f / static_cast<float>(n_proj_ffs);
-    float lower = mn[p], upper = mx[p];
-    float T = (mx[p] + mn[p]) / 2.0f;
-    const float lhs = 0.03f * strength * ecc[p];
+    float lower = lo, upper = hi;
+    float T = (hi + lo) / 2.0f;
+    const float lhs = 0.03f * strength * ecc;
     float rhs = af_rhs_wave(win, numel, T, inv_pi, inv_npf);
     int iter = 0;
     while (static_cast<double>(fabsf(lhs - rhs) / lhs) > 0.01) {
@@ -237,7 +225,8 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     if (a.n_proj > 65535 || a.n_rows_raw > 65535) return CTBB_ERR_SHAPE;  // grid.z / grid.y limits
     if ((static_cast<size_t>(a.n_rows_raw) * a.nch) % 4 != 0) return CTBB_ERR_SHAPE;  // float4 frames
     const size_t np = static_cast<size_t>(a.n_proj);
-    float *sup = a.work, *sm = sup + np, *mx = sm + np, *mn = mx + np, *ecc = mn + np, *thr = ecc + np;
+    // work: [sup][sm][3 unused][thr], n_proj floats each
+    float *sup = a.work, *sm = sup + np, *thr = sm + 4 * np;
     const size_t frame = static_cast<size_t>(a.n_rows_raw) * a.nch;
     const int n_avg = odd_window(18.0f / 360.0f, a.n_proj_ffs);
     const int n_range = odd_window(180.0f / 360.0f, a.n_proj_ffs);
@@ -248,15 +237,14 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     } else {
         hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
         hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
-        hipLaunchKernelGGL(af_ecc, dim3((a.n_proj + 3) / 4), dim3(256), 0, stream, sm, mx, mn, ecc, a.n_proj, n_range);
         // the counting window (at most n_range values) in LDS when it takes <= 16 KB
         const size_t win_bytes = static_cast<size_t>(n_range) * sizeof(float);
         if (win_bytes <= 16384)
-            hipLaunchKernelGGL(af_thresh<true>, dim3(a.n_proj), dim3(64), win_bytes, stream, sm, mx, mn, ecc, thr,
-                               a.n_proj, n_range, a.n_proj_ffs, a.strength);
-        else
-            hipLaunchKernelGGL(af_thresh<false>, dim3(a.n_proj), dim3(64), 0, stream, sm, mx, mn, ecc, thr, a.n_proj,
+            hipLaunchKernelGGL(af_thresh<true>, dim3(a.n_proj), dim3(64), win_bytes, stream, sm, thr, a.n_proj,
                                n_range, a.n_proj_ffs, a.strength);
+        else
+            hipLaunchKernelGGL(af_thresh<false>, dim3(a.n_proj), dim3(64), 0, stream, sm, thr, a.n_proj, n_range,
+                               a.n_proj_ffs, a.strength);
     }
     const bool aligned = (reinterpret_cast<uintptr_t>(a.raw) & 15u) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 15u) == 0;
     if (a.vec4 && aligned)
