@@ -152,8 +152,12 @@ __device__ __forceinline__ float af_value(const float *__restrict__ raw, float x
                                           int n_proj, int n_rows_raw, int nch, size_t ps) {
     if (!(x > T)) return x;
     float norm = 0.0f, val = 0.0f;
+    // (outer loops rolled: few registers, so the kernels fit beside a
+    // backprojection; the order of the sums is unchanged)
+#pragma unroll 1
     for (int ii = 0; ii < 5; ii++) {
         const int fc = i + ii - 2;
+#pragma unroll 1
         for (int jj = 0; jj < 5; jj++) {
             const int fr = j + jj - 2;
             for (int kk = 0; kk < 5; kk++) {
@@ -206,6 +210,26 @@ __global__ __launch_bounds__(256) void af_apply4(const float *raw, const float *
     *reinterpret_cast<float4 *>(out + base) = y;
 }
 
+// Only the views the filter changes: view p is unchanged where its maximum
+// (af_sup, from 0) is at or below its threshold, since af_value keeps every
+// sample x <= T.  One workgroup per view flags it (hot[p]) and, if changed,
+// writes its filtered frame; a reader takes the other views from the raw
+// pull.  (C1: about 6% of the views.)
+__global__ __launch_bounds__(256) void af_apply_hot(const float *raw, const float *sup, const float *thr, float *out,
+                                                    unsigned char *hot, int n_proj, int n_rows_raw, int nch) {
+    const int p = blockIdx.x;
+    const float T = thr[p];
+    const bool h = sup[p] > T;
+    if (threadIdx.x == 0) hot[p] = h ? 1 : 0;
+    if (!h) return;
+    const size_t ps = static_cast<size_t>(nch) * n_rows_raw;
+    for (int j = 0; j < n_rows_raw; j++)
+        for (int i = threadIdx.x; i < nch; i += 256) {
+            const size_t idx = ps * p + static_cast<size_t>(j) * nch + i;
+            out[idx] = af_value(raw, raw[idx], T, i, j, p, n_proj, n_rows_raw, nch, ps);
+        }
+}
+
 // Fixed-threshold mode: every view's threshold is the same constant.
 __global__ __launch_bounds__(256) void af_const(float *thr, int n_proj, float T) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -234,6 +258,8 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     if (a.fixed_threshold > 0.0f) {
         if (!(a.fixed_threshold < 3.0e38f)) return CTBB_ERR_ARG;
         hipLaunchKernelGGL(af_const, dim3(g1), dim3(256), 0, stream, thr, a.n_proj, a.fixed_threshold);
+        // (the view maxima decide which views hot_only writes)
+        if (a.hot_only) hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
     } else {
         hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
         hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
@@ -245,6 +271,11 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
         else
             hipLaunchKernelGGL(af_thresh<false>, dim3(a.n_proj), dim3(64), 0, stream, sm, thr, a.n_proj, n_range,
                                a.n_proj_ffs, a.strength);
+    }
+    if (a.hot_only) {
+        hipLaunchKernelGGL(af_apply_hot, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, thr, a.out,
+                           adaptive_hot_flags(a.work, a.n_proj), a.n_proj, a.n_rows_raw, a.nch);
+        return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
     }
     const bool aligned = (reinterpret_cast<uintptr_t>(a.raw) & 15u) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 15u) == 0;
     if (a.vec4 && aligned)

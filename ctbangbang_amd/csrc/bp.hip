@@ -460,24 +460,40 @@ __device__ __forceinline__ void bp_tile(Args &a, int tile, int zblk, float2 *lds
     }
 }
 
-// One launch per block: workgroup = tile (x) and slice range (y).
+// One launch per block: slice range blockIdx.y; a grid of at most the
+// resident workgroups (a multiple of 8 where n_tiles is), workgroup b taking
+// tiles b, b + gridDim.x, ... — all of them placed at launch, so kernels
+// started beside the backprojection cannot take the registers a later
+// workgroup of it would need.  XCD-aware remap: hardware dispatches workgroup
+// b to XCD b % 8, and item i maps to tile (i % 8) per + i / 8, so each XCD
+// works through a contiguous band of tiles.
 template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
 __device__ __forceinline__ void bp_block_body(const BpArgs &a, float2 *lds2) {
-    // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give each
-    // XCD a contiguous range of tiles.
-    int tile = blockIdx.x;
-    if ((a.n_tiles & 7) == 0) {
-        const int per = a.n_tiles >> 3;
-        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int per = (a.n_tiles & 7) == 0 ? a.n_tiles >> 3 : 0;
+    for (int item = blockIdx.x; item < a.n_tiles; item += gridDim.x) {
+        const int tile = per ? (item & 7) * per + (item >> 3) : item;
+        if (item != static_cast<int>(blockIdx.x)) __syncthreads();   // LDS reuse by the next tile
+        bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, blockIdx.y, lds2);
     }
-    bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, blockIdx.y, lds2);
 }
 
-// Two z-groups (512 threads, 4 waves per SIMD; at most 128 VGPRs, 120 for
-// the usual shapes, which leaves room for the next block's low-register
-// preparation kernels beside it).
+// Two z-groups (512 threads, 4 waves per SIMD).  VGPR limits (the
+// attribute counts register pairs on gfx950): 112 registers for windows of
+// up to 8 channels per staging thread (bp_kernel2), so the 4 waves per SIMD
+// leave 64 per lane to the next block's preparation kernels, which then run
+// beside the backprojection (at 120 its 4 waves hold the whole register
+// file and the preparation only gets the registers of retiring workgroups;
+// the values the cap spills are reloaded once per tile); 120 for the wider
+// windows (bp_kernel2w), whose prefetch registers would spill in the view
+// loop.
 template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void bp_kernel2(BpArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4), amdgpu_num_vgpr(56))) void bp_kernel2(BpArgs a) {
+    extern __shared__ float2 lds2[];
+    bp_block_body<NZ, CPT, TPR, ZS, kBf16>(a, lds2);
+}
+
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4), amdgpu_num_vgpr(60))) void bp_kernel2w(BpArgs a) {
     extern __shared__ float2 lds2[];
     bp_block_body<NZ, CPT, TPR, ZS, kBf16>(a, lds2);
 }
@@ -492,7 +508,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
 template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
 constexpr auto bp_kernel_of() {
     if constexpr (ZS == 1) return &bp_kernel1<NZ, CPT, TPR, ZS, kBf16>;
-    else return &bp_kernel2<NZ, CPT, TPR, ZS, kBf16>;
+    else if constexpr (CPT <= 8) return &bp_kernel2<NZ, CPT, TPR, ZS, kBf16>;
+    else return &bp_kernel2w<NZ, CPT, TPR, ZS, kBf16>;
 }
 
 // Persistent form: a grid of at most the resident workgroups works through
@@ -531,20 +548,34 @@ __device__ __forceinline__ void bp_persistent_body(const BpQueues &Q, float2 *ld
     }
 }
 
-// Two z-groups (512 threads): at most 120 VGPRs (the attribute counts VGPR
-// pairs on gfx950), so the next group's low-register preparation kernels
-// fit beside it.
+// Two z-groups (512 threads), the VGPR limits of bp_kernel2 / bp_kernel2w
+// (the next group's low-register preparation kernels fit beside it).
 template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) __attribute__((amdgpu_num_vgpr(60)))
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) __attribute__((amdgpu_num_vgpr(56)))
 void bp_persistent2_kernel(BpQueues Q) {
     extern __shared__ float2 lds2[];
     bp_persistent_body<NZ, CPT, TPR, ZS, kBf16>(Q, lds2);
 }
 
 template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) __attribute__((amdgpu_num_vgpr(60)))
+void bp_persistent2w_kernel(BpQueues Q) {
+    extern __shared__ float2 lds2[];
+    bp_persistent_body<NZ, CPT, TPR, ZS, kBf16>(Q, lds2);
+}
+
+
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bp_persistent1_kernel(BpQueues Q) {
     extern __shared__ float2 lds2[];
     bp_persistent_body<NZ, CPT, TPR, ZS, kBf16>(Q, lds2);
+}
+
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+constexpr auto bp_persistent_of() {
+    if constexpr (ZS == 1) return &bp_persistent1_kernel<NZ, CPT, TPR, ZS, kBf16>;
+    else if constexpr (CPT <= 8) return &bp_persistent2_kernel<NZ, CPT, TPR, ZS, kBf16>;
+    else return &bp_persistent2w_kernel<NZ, CPT, TPR, ZS, kBf16>;
 }
 
 // View table of one block, built on the device (the backprojection's work
@@ -589,7 +620,18 @@ int launch_c(const BpArgs &a, hipStream_t stream) {
                            : reinterpret_cast<const void *>(bp_kernel_of<NZ, CPT, TPR, ZS, false>());
     const int st = allow_lds(kf, lds_bytes);
     if (st) return st;
-    dim3 grid(a.n_tiles, 32 / (NZ * ZS));
+    // the resident workgroups (per slice range), a multiple of 8 where the
+    // XCD remap applies
+    int dev = 0, n_cu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 256 * ZS, lds_bytes) != hipSuccess)
+        return CTBB_ERR_HIP;
+    const int ny = 32 / (NZ * ZS);
+    long gx = std::max(1, per_cu) * static_cast<long>(std::max(1, n_cu)) / ny;
+    if ((a.n_tiles & 7) == 0) gx = std::max<long>(8, gx / 8 * 8);
+    gx = std::min<long>(gx, a.n_tiles);
+    dim3 grid(static_cast<unsigned>(gx), ny);
     void *args[] = {const_cast<BpArgs *>(&a)};
     return hipLaunchKernel(kf, grid, dim3(256 * ZS), args, lds_bytes, stream) == hipSuccess ? CTBB_OK
                                                                                            : CTBB_ERR_HIP;
@@ -598,13 +640,8 @@ int launch_c(const BpArgs &a, hipStream_t stream) {
 template <int NZ, int CPT, int TPR, int ZS>
 int launch_p(const BpArgs &a, const BpQueues &Q, int total_items, int max_groups, hipStream_t stream) {
     const size_t lds_bytes = lds_bytes_of<NZ, CPT, TPR, ZS>(a);
-    const void *kf;
-    if constexpr (ZS == 2)
-        kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_persistent2_kernel<NZ, CPT, TPR, ZS, true>)
-                          : reinterpret_cast<const void *>(&bp_persistent2_kernel<NZ, CPT, TPR, ZS, false>);
-    else
-        kf = a.rebin_bf16 ? reinterpret_cast<const void *>(&bp_persistent1_kernel<NZ, CPT, TPR, ZS, true>)
-                          : reinterpret_cast<const void *>(&bp_persistent1_kernel<NZ, CPT, TPR, ZS, false>);
+    const void *kf = a.rebin_bf16 ? reinterpret_cast<const void *>(bp_persistent_of<NZ, CPT, TPR, ZS, true>())
+                                  : reinterpret_cast<const void *>(bp_persistent_of<NZ, CPT, TPR, ZS, false>());
     int st = allow_lds(kf, lds_bytes);
     if (st) return st;
     int dev = 0, n_cu = 0, per_cu = 0;
@@ -616,22 +653,11 @@ int launch_p(const BpArgs &a, const BpQueues &Q, int total_items, int max_groups
     if (max_groups > 0) groups = std::min<long>(groups, max_groups);
     groups = std::min<long>(groups, std::max(total_items, 1));
     groups = std::max<long>(8, (groups + 7) / 8 * 8);     // every queue has workgroups starting on it
-    if constexpr (ZS == 2) {
-        if (a.rebin_bf16)
-            hipLaunchKernelGGL((bp_persistent2_kernel<NZ, CPT, TPR, ZS, true>), dim3(static_cast<unsigned>(groups)),
-                               dim3(256 * ZS), lds_bytes, stream, Q);
-        else
-            hipLaunchKernelGGL((bp_persistent2_kernel<NZ, CPT, TPR, ZS, false>), dim3(static_cast<unsigned>(groups)),
-                               dim3(256 * ZS), lds_bytes, stream, Q);
-    } else {
-        if (a.rebin_bf16)
-            hipLaunchKernelGGL((bp_persistent1_kernel<NZ, CPT, TPR, ZS, true>), dim3(static_cast<unsigned>(groups)),
-                               dim3(256 * ZS), lds_bytes, stream, Q);
-        else
-            hipLaunchKernelGGL((bp_persistent1_kernel<NZ, CPT, TPR, ZS, false>), dim3(static_cast<unsigned>(groups)),
-                               dim3(256 * ZS), lds_bytes, stream, Q);
-    }
-    return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
+    void *args[] = {const_cast<BpQueues *>(&Q)};
+    return hipLaunchKernel(kf, dim3(static_cast<unsigned>(groups)), dim3(256 * ZS), args, lds_bytes, stream) ==
+                   hipSuccess
+               ? CTBB_OK
+               : CTBB_ERR_HIP;
 }
 
 // Kernel variant of a block: slices per thread (NZ, ZS), channels per

@@ -63,6 +63,11 @@ struct RebinArgs {
     int n_ffs, nch, nos, n_rows_raw, n_rows, n_proj, n_proj_out, add;
     int reverse_rows;
     unsigned *nonfinite;       // set to 1 when an output sample is not finite (or null)
+    // optional (null: every frame from raw): hot[f] != 0 reads frame f of the
+    // pull from raw_af instead (the adaptive filter's output, written only
+    // for the frames it changes)
+    const unsigned char *hot;
+    const float *raw_af;
 };
 int launch_rebin(const RebinArgs &a, hipStream_t stream);
 
@@ -92,7 +97,15 @@ struct AdaptiveArgs {
     float strength;
     int vec4;                  // 1: 16-byte apply kernel where aligned (GPU otherwise idle)
     float fixed_threshold;     // > 0: this threshold for every view instead of the KK bisection
+    // 1: write out only for the views the filter changes (those whose maximum
+    // exceeds their threshold), flagged in adaptive_hot_flags(work, n_proj);
+    // a reader takes the other views from raw (RebinArgs::hot)
+    int hot_only;
 };
+// the per-view flags of hot_only inside AdaptiveArgs::work (n_proj bytes)
+inline unsigned char *adaptive_hot_flags(float *work, int n_proj) {
+    return reinterpret_cast<unsigned char *>(work + 2 * static_cast<size_t>(n_proj));
+}
 int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream);
 
 // ---------------------------------------------------------------------------

@@ -51,12 +51,20 @@ __device__ __forceinline__ void report_nonfinite(unsigned *flag, bool bad) {
         *reinterpret_cast<volatile unsigned *>(flag) = 1u;
 }
 
+// Base of frame f of the pull: with kHot the adaptive filter's output where
+// it changed the frame, the raw pull otherwise.
+template <bool kHot>
+__device__ __forceinline__ const float *frame_src(const RebinArgs &a, size_t f) {
+    if (!kHot) return a.raw;
+    return a.hot[f] ? a.raw_af : a.raw;
+}
+
+template <bool kHot>
 __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (c >= a.nos) return;
     const size_t frame = static_cast<size_t>(a.n_rows_raw) * a.nch;
-    const float *__restrict__ raw = a.raw;
     const float *__restrict__ aoff = a.aoff;
     float *__restrict__ outp = a.out;
 
@@ -96,9 +104,9 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
             int p0, p1;
             float fa;
             taps(proj - ao0, a.n_proj, p0, p1, fa);
-            const float *row = raw + static_cast<size_t>(j) * a.nch;
-            const float *s0 = row + (static_cast<size_t>(p0) * a.n_ffs + fs) * frame;
-            const float *s1 = row + (static_cast<size_t>(p1) * a.n_ffs + fs) * frame;
+            const size_t f0 = static_cast<size_t>(p0) * a.n_ffs + fs, f1 = static_cast<size_t>(p1) * a.n_ffs + fs;
+            const float *s0 = frame_src<kHot>(a, f0) + static_cast<size_t>(j) * a.nch + f0 * frame;
+            const float *s1 = frame_src<kHot>(a, f1) + static_cast<size_t>(j) * a.nch + f1 * frame;
             const float t0 = fmaf(fc, s0[c1] - s0[c0], s0[c0]);
             const float t1 = fmaf(fc, s1[c1] - s1[c0], s1[c0]);
             out = fmaf(fa, t1 - t0, t0);
@@ -114,9 +122,9 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
                 int p0, p1;
                 float fa;
                 taps(proj - (h ? ao1 : ao0), a.n_proj, p0, p1, fa);
-                const float *row = raw + static_cast<size_t>(j) * a.nch + ch;
-                const float u0 = row[(static_cast<size_t>(p0) * a.n_ffs + fs) * frame];
-                const float u1 = row[(static_cast<size_t>(p1) * a.n_ffs + fs) * frame];
+                const size_t f0 = static_cast<size_t>(p0) * a.n_ffs + fs, f1 = static_cast<size_t>(p1) * a.n_ffs + fs;
+                const float u0 = frame_src<kHot>(a, f0)[static_cast<size_t>(j) * a.nch + ch + f0 * frame];
+                const float u1 = frame_src<kHot>(a, f1)[static_cast<size_t>(j) * a.nch + ch + f1 * frame];
                 t[h] = fmaf(fa, u1 - u0, u0);
             }
             out = fmaf(fm, t[1] - t[0], t[0]);
@@ -137,7 +145,15 @@ __device__ __forceinline__ float ld(const float *base, unsigned boff) {
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff);
 }
 
-template <bool kSplit>
+// Sample at byte offset boff of frame f (kHot: from the adaptive filter's
+// output where it changed frame f).
+template <bool kHot>
+__device__ __forceinline__ float ld_frame(const RebinArgs &a, unsigned f, unsigned boff) {
+    if (!kHot) return ld(a.raw, boff);
+    return ld(a.hot[f] ? a.raw_af : a.raw, boff);
+}
+
+template <bool kSplit, bool kHot>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void rebin32_kernel(RebinArgs a) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
@@ -177,10 +193,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void r
             int p0, p1;
             float fa;
             taps(proj - ao0, a.n_proj, p0, p1, fa);
-            const unsigned b0 = (static_cast<unsigned>(p0 * a.n_ffs) + fs) * frame_b + row_b;
-            const unsigned b1 = (static_cast<unsigned>(p1 * a.n_ffs) + fs) * frame_b + row_b;
-            const float s00 = ld(a.raw, b0 + 4u * c0), s01 = ld(a.raw, b0 + 4u * c1);
-            const float s10 = ld(a.raw, b1 + 4u * c0), s11 = ld(a.raw, b1 + 4u * c1);
+            const unsigned f0 = static_cast<unsigned>(p0 * a.n_ffs) + fs, f1 = static_cast<unsigned>(p1 * a.n_ffs) + fs;
+            const unsigned b0 = f0 * frame_b + row_b, b1 = f1 * frame_b + row_b;
+            const float s00 = ld_frame<kHot>(a, f0, b0 + 4u * c0), s01 = ld_frame<kHot>(a, f0, b0 + 4u * c1);
+            const float s10 = ld_frame<kHot>(a, f1, b1 + 4u * c0), s11 = ld_frame<kHot>(a, f1, b1 + 4u * c1);
             const float t0 = fmaf(fc, s01 - s00, s00);
             const float t1 = fmaf(fc, s11 - s10, s10);
             out = fmaf(fa, t1 - t0, t0);
@@ -194,8 +210,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void r
                 int p0, p1;
                 float fa;
                 taps(proj - (h ? ao1 : ao0), a.n_proj, p0, p1, fa);
-                const float u0 = ld(a.raw, (static_cast<unsigned>(p0 * a.n_ffs) + fs) * frame_b + ch_b);
-                const float u1 = ld(a.raw, (static_cast<unsigned>(p1 * a.n_ffs) + fs) * frame_b + ch_b);
+                const unsigned f0 = static_cast<unsigned>(p0 * a.n_ffs) + fs, f1 = static_cast<unsigned>(p1 * a.n_ffs) + fs;
+                const float u0 = ld_frame<kHot>(a, f0, f0 * frame_b + ch_b);
+                const float u1 = ld_frame<kHot>(a, f1, f1 * frame_b + ch_b);
                 t[h] = fmaf(fa, u1 - u0, u0);
             }
             out = fmaf(fc, t[1] - t[0], t[0]);
@@ -210,7 +227,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void r
 }  // namespace
 
 int launch_rebin(const RebinArgs &a, hipStream_t stream) {
-    if (!a.raw || !a.out || !a.bidx || !a.aoff) return CTBB_ERR_ARG;
+    if (!a.raw || !a.out || !a.bidx || !a.aoff || (a.hot && !a.raw_af)) return CTBB_ERR_ARG;
     if (a.kind < 0 || a.kind > 3 || a.nch <= 1 || a.nos != 2 * a.nch || a.n_proj < 2 || a.n_proj_out <= 0)
         return CTBB_ERR_SHAPE;
     if (a.add < 0 || a.add + a.n_proj_out > a.n_proj) return CTBB_ERR_SHAPE;
@@ -221,12 +238,20 @@ int launch_rebin(const RebinArgs &a, hipStream_t stream) {
     const unsigned long long raw_bytes = static_cast<unsigned long long>(a.n_proj) * a.n_ffs * a.n_rows_raw * a.nch * 4ull;
     const unsigned long long out_bytes = static_cast<unsigned long long>(a.n_proj_out) * a.n_rows * a.nos * 4ull;
     if (raw_bytes < (1ull << 32) && out_bytes < (1ull << 32)) {
-        if (a.kind <= 1)
-            hipLaunchKernelGGL(rebin32_kernel<false>, grid, dim3(256), 0, stream, a);
+        const bool hot = a.hot != nullptr;
+        if (a.kind <= 1 && !hot)
+            hipLaunchKernelGGL((rebin32_kernel<false, false>), grid, dim3(256), 0, stream, a);
+        else if (a.kind <= 1)
+            hipLaunchKernelGGL((rebin32_kernel<false, true>), grid, dim3(256), 0, stream, a);
+        else if (!hot)
+            hipLaunchKernelGGL((rebin32_kernel<true, false>), grid, dim3(256), 0, stream, a);
         else
-            hipLaunchKernelGGL(rebin32_kernel<true>, grid, dim3(256), 0, stream, a);
+            hipLaunchKernelGGL((rebin32_kernel<true, true>), grid, dim3(256), 0, stream, a);
     } else {
-        hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, stream, a);
+        if (a.hot)
+            hipLaunchKernelGGL(rebin_kernel<true>, grid, dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL(rebin_kernel<false>, grid, dim3(256), 0, stream, a);
     }
     return hipGetLastError() == hipSuccess ? CTBB_OK : CTBB_ERR_HIP;
 }
