@@ -514,37 +514,56 @@ constexpr auto bp_kernel_of() {
 
 // Persistent form: a grid of at most the resident workgroups works through
 // the (block, tile) items of many blocks (jobs).  Eight queues, one per XCD
-// (workgroup b runs on XCD b % 8, where the hardware dispatches it): queue q
-// holds band q of every job, the contiguous tiles
+// (workgroup b starts on queue b % 8, the XCD the hardware dispatches it
+// to): queue q holds band q of every job, the contiguous tiles
 // [floor(q n / 8), floor((q + 1) n / 8)) of its n tiles, in job order; a
-// launch covers the items of jobs [job_first, job_end).  The
-// g = gridDim / 8 workgroups of queue q take its items in a fixed stride
-// (workgroup b: items b / 8, b / 8 + g, ...), so the g tiles an XCD runs at
-// one time lie side by side and share its L2, as in one launch per block.
-// (Items handed out by an atomic counter instead were measured 18% slower
-// on C1.)  Every workgroup returns after its last item.
+// launch covers the items of jobs [job_first, job_end).  A workgroup takes
+// items one at a time from its queue's counter, so the tiles an XCD runs at
+// one time lie side by side and share its L2, as in one launch per block;
+// when its queue is empty it takes what is left of the other queues in turn,
+// and returns when all eight are empty.  Handing items out this way keeps
+// the workgroups level: the preparation kernels that run beside a long
+// launch slow the workgroups of some CUs more than others, and with fixed
+// shares the slowest ran alone for the last 10-15% of a launch over 10-20
+// blocks.  (The counter's answer is passed through dynamic LDS: a static LDS
+// variable would move the dynamic base off the 16-byte alignment its 128-bit
+// accesses need, which cost 12% here.)
+template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
+__device__ __forceinline__ void bp_persistent_tile(const BpQueues &Q, const int *pre, int q, int item,
+                                                   float2 *lds2) {
+    // the job holding this item: the last j with pre[j] <= item
+    int lo = Q.job_first, hi = Q.job_end - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (__builtin_amdgcn_readfirstlane(pre[mid]) <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    // the job through the scalar cache: the plan kernels wrote the jobs
+    // before this launch and nothing writes them during it
+    const __attribute__((address_space(4))) BpArgs &a =
+        *(const __attribute__((address_space(4))) BpArgs *)(Q.jobs + lo);
+    const int tile = (q * a.n_tiles) / 8 + (item - __builtin_amdgcn_readfirstlane(pre[lo]));
+    bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, 0, lds2);
+}
+
 template <int NZ, int CPT, int TPR, int ZS, bool kBf16>
 __device__ __forceinline__ void bp_persistent_body(const BpQueues &Q, float2 *lds2) {
-    const int q = blockIdx.x & 7;
-    const int per_q = gridDim.x >> 3;
-    const int *pre = Q.band_pre + q * (Q.n_jobs + 1);
-    const int len = __builtin_amdgcn_readfirstlane(pre[Q.job_end]);
-    for (int item = __builtin_amdgcn_readfirstlane(pre[Q.job_first]) + (blockIdx.x >> 3); item < len;
-         item += per_q) {
-        // the job holding this item: the last j with pre[j] <= item
-        int lo = Q.job_first, hi = Q.job_end - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (__builtin_amdgcn_readfirstlane(pre[mid]) <= item) lo = mid;
-            else hi = mid - 1;
+    int *slot = reinterpret_cast<int *>(reinterpret_cast<char *>(lds2) + Q.slot_offset);   // double-buffered
+    const int q0 = blockIdx.x & 7;
+    int par = 0;
+    for (int k = 0; k < 8; k++) {
+        const int q = (q0 + k) & 7;
+        const int *pre = Q.band_pre + q * (Q.n_jobs + 1);
+        const int first = __builtin_amdgcn_readfirstlane(pre[Q.job_first]);
+        const int len = __builtin_amdgcn_readfirstlane(pre[Q.job_end]);
+        for (;;) {
+            if (threadIdx.x == 0) slot[par] = first + atomicAdd(Q.counters + q, 1);
+            __syncthreads();                    // slot written; every thread has left the previous tile
+            const int item = __builtin_amdgcn_readfirstlane(slot[par]);
+            par ^= 1;
+            if (item >= len) break;
+            bp_persistent_tile<NZ, CPT, TPR, ZS, kBf16>(Q, pre, q, item, lds2);
         }
-        // the job through the scalar cache: the plan kernels wrote the jobs
-        // before this launch and nothing writes them during it
-        const __attribute__((address_space(4))) BpArgs &a =
-            *(const __attribute__((address_space(4))) BpArgs *)(Q.jobs + lo);
-        const int tile = (q * a.n_tiles) / 8 + (item - __builtin_amdgcn_readfirstlane(pre[lo]));
-        __syncthreads();                        // every thread has left the previous tile (LDS reuse)
-        bp_tile<NZ, CPT, TPR, ZS, kBf16>(a, tile, 0, lds2);
     }
 }
 
@@ -642,19 +661,22 @@ int launch_p(const BpArgs &a, const BpQueues &Q, int total_items, int max_groups
     const size_t lds_bytes = lds_bytes_of<NZ, CPT, TPR, ZS>(a);
     const void *kf = a.rebin_bf16 ? reinterpret_cast<const void *>(bp_persistent_of<NZ, CPT, TPR, ZS, true>())
                                   : reinterpret_cast<const void *>(bp_persistent_of<NZ, CPT, TPR, ZS, false>());
-    int st = allow_lds(kf, lds_bytes);
+    BpQueues QL = Q;
+    QL.slot_offset = static_cast<int>((lds_bytes + 15) & ~static_cast<size_t>(15));
+    const size_t lds_total = QL.slot_offset + 16;               // + the counter slots
+    int st = allow_lds(kf, lds_total);
     if (st) return st;
     int dev = 0, n_cu = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 256 * ZS, lds_bytes) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 256 * ZS, lds_total) != hipSuccess)
         return CTBB_ERR_HIP;
     long groups = static_cast<long>(std::max(per_cu, 1)) * std::max(n_cu, 1);
     if (max_groups > 0) groups = std::min<long>(groups, max_groups);
     groups = std::min<long>(groups, std::max(total_items, 1));
     groups = std::max<long>(8, (groups + 7) / 8 * 8);     // every queue has workgroups starting on it
-    void *args[] = {const_cast<BpQueues *>(&Q)};
-    return hipLaunchKernel(kf, dim3(static_cast<unsigned>(groups)), dim3(256 * ZS), args, lds_bytes, stream) ==
+    void *args[] = {&QL};
+    return hipLaunchKernel(kf, dim3(static_cast<unsigned>(groups)), dim3(256 * ZS), args, lds_total, stream) ==
                    hipSuccess
                ? CTBB_OK
                : CTBB_ERR_HIP;
@@ -783,7 +805,7 @@ int launch_backproject(const BpArgs &a0, hipStream_t stream) {
 
 int launch_backproject_persistent(const BpArgs &sample, int variant, const BpQueues &Q, int total_items,
                                   int max_groups, hipStream_t stream) {
-    if (!Q.jobs || !Q.band_pre || Q.n_jobs <= 0 || Q.job_first < 0 || Q.job_end <= Q.job_first ||
+    if (!Q.jobs || !Q.band_pre || !Q.counters || Q.n_jobs <= 0 || Q.job_first < 0 || Q.job_end <= Q.job_first ||
         Q.job_end > Q.n_jobs || variant < 0)
         return CTBB_ERR_ARG;
     if (total_items <= 0) return CTBB_OK;

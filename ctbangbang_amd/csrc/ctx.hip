@@ -123,8 +123,9 @@ struct ctbb_ctx {
     // batching form: filtered-projection and image arenas per group parity,
     // the device work list (plans, jobs, view tables, queues), and the
     // executable graphs of launched group preparations (kept until the
-    // context's work has finished)
-    DevBuf arena[2], img_arena[2], plan_buf, tpos_buf;
+    // context's work has finished), the item counters of the persistent
+    // backprojection launches (8 per launch)
+    DevBuf arena[2], img_arena[2], plan_buf, tpos_buf, bp_counters;
     hipEvent_t ev_gdone[2] = {nullptr, nullptr}, ev_gprep = nullptr;
     std::vector<hipGraphExec_t> live_graphs;
     bool capturing = false;      // a graph is being captured: no event timing
@@ -790,7 +791,8 @@ void ctbb_ctx_destroy(ctbb_ctx *c) {
     if (c->stream_head) (void)hipStreamDestroy(c->stream_head);
     for (hipGraphExec_t g : c->live_graphs) (void)hipGraphExecDestroy(g);
     c->live_graphs.clear();
-    for (DevBuf *b : {&c->arena[0], &c->arena[1], &c->img_arena[0], &c->img_arena[1], &c->plan_buf, &c->tpos_buf})
+    for (DevBuf *b : {&c->arena[0], &c->arena[1], &c->img_arena[0], &c->img_arena[1], &c->plan_buf, &c->tpos_buf,
+                      &c->bp_counters})
         b->release();
     for (hipEvent_t e : {c->ev_gdone[0], c->ev_gdone[1], c->ev_gprep})
         if (e) (void)hipEventDestroy(e);
@@ -1513,10 +1515,17 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
                                reinterpret_cast<int *>(dplan + lay[g].band_pre), c->stream);
         if (st) return st;
     }
+    // the item counters of every persistent launch (the first group runs one
+    // launch per block, every later group one), zeroed behind the work list
+    const size_t n_launch = static_cast<size_t>(groups[0].n_blocks) + groups.size() - 1;
+    if ((st = c->bp_counters.ensure(8 * n_launch * sizeof(int))) ||
+        (st = fill_zero(c->bp_counters.as<float>(), 8 * n_launch, c->stream)))
+        return st;
     // the backprojection stream starts after the work list
     if (hipEventRecord(c->ev_sync, c->stream) != hipSuccess || hipStreamWaitEvent(c->stream_bp, c->ev_sync, 0) != hipSuccess)
         return CTBB_ERR_HIP;
     const bool split = c->opt.proj_dtype != CTBB_PROJ_BF16 && c->opt.filter_mode == CTBB_FILTER_FFT32;
+    size_t launch = 0;
     struct JobRef {
         int s, b;
         size_t slot;        // element offset of the block's filtered projections in the arena
@@ -1613,6 +1622,7 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
             Q.n_jobs = groups[g].n_blocks;
             Q.job_first = rg.first;
             Q.job_end = rg.second;
+            Q.counters = c->bp_counters.as<int>() + 8 * launch++;
             int total = 0;
             for (int j = rg.first; j < rg.second; j++) total += info[jobs[j].s].tmpl.n_tiles;
             EventPair eb2 = c->begin("backproject", c->stream_bp);

@@ -41,6 +41,8 @@ struct BpQueues {
     const int *band_pre;       // [8][n_jobs + 1]: items of queue q before job j (band q of each job)
     int n_jobs;
     int job_first, job_end;    // the launch covers jobs [job_first, job_end)
+    int *counters;             // [8] item counters, one per queue, zeroed before the launch
+    int slot_offset;           // set by the launcher: byte offset of the counter slots in dynamic LDS
 };
 // Fills the staging shape (tpr, tpr_log2) of a and returns an identifier of
 // its kernel variant (equal identifiers: one persistent launch may hold
