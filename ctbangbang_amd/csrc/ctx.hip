@@ -437,7 +437,7 @@ int fill_bp_args(const recon_metadata *mr, BpArgs &a) {
 }
 
 int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin, const BpView *d_vtab,
-                   float *d_out, hipStream_t on = nullptr, bool timed = true) {
+                   float *d_out, hipStream_t on = nullptr, bool timed = true, bool beside = false) {
     if (!on) on = ctx->stream;
     BpArgs a{};
     int st = fill_bp_args(mr, a);
@@ -447,9 +447,9 @@ int do_backproject(ctbb_ctx *ctx, const recon_metadata *mr, const void *d_rebin,
     else a.rebin = static_cast<const float *>(d_rebin);
     a.vtab = d_vtab;
     a.out = d_out;
-    if (!timed) return launch_backproject(a, on);
+    if (!timed) return launch_backproject(a, on, beside);
     EventPair e = ctx->begin("backproject", on);
-    st = launch_backproject(a, on);
+    st = launch_backproject(a, on, beside);
     ctx->end(e);
     return st;
 }
@@ -703,7 +703,7 @@ int pipeline_block(ctbb_ctx *c, recon_metadata *mr, int item, int b, const float
         return CTBB_ERR_HIP;
     const size_t nxy = static_cast<size_t>(mr->rp.nx) * mr->rp.ny;
     st = do_backproject(c, mr, out.p, d_vtab, d_image + static_cast<size_t>(b) * mr->ri.n_slices_block * nxy,
-                        c->stream_bp);
+                        c->stream_bp, true, /*beside: the next block's preparation*/ true);
     if (st) return st;
     return hip_ok(hipEventRecord(c->ev_bp[k], c->stream_bp));
 }

@@ -31,7 +31,10 @@ struct BpArgs {
     float half_w;              // footprint half width of a 16x16 tile, channels
     float acq_half, out_scale;
 };
-int launch_backproject(const BpArgs &a, hipStream_t stream);
+// beside: the caller schedules other kernels to run beside this launch
+// (then the grid is only the resident workgroups, which stride over the
+// tiles; otherwise one workgroup per tile).
+int launch_backproject(const BpArgs &a, hipStream_t stream, bool beside = false);
 
 // Persistent backprojection over the tiles of many blocks (jobs).  The jobs
 // of one launch share the kernel variant (rows, staging shape, slices per
