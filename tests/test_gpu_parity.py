@@ -553,9 +553,14 @@ def test_reconstruct_host_matches_dev(ctx, tmp_path, mini_dir, ffs):
     cb.free_metadata(mr)
 
 
-@pytest.mark.parametrize("mode,group_mb", [(cb.BATCH_PERSISTENT, 0), (cb.BATCH_PERSISTENT, 1),
-                                           (cb.BATCH_PIPELINE, 0)])
-def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir, mode, group_mb):
+@pytest.mark.parametrize("mode,group_mb,filt,proj", [
+    (cb.BATCH_PERSISTENT, 0, cb.FILTER_FFT32, cb.PROJ_F32), (cb.BATCH_PERSISTENT, 1, cb.FILTER_FFT32, cb.PROJ_F32),
+    (cb.BATCH_PIPELINE, 0, cb.FILTER_FFT32, cb.PROJ_F32),
+    # BF16 projections or the matrix-core filter: the preparation graph
+    # captures the whole per-block preparation, and the persistent kernel
+    # reads BF16 windows
+    (cb.BATCH_PERSISTENT, 1, cb.FILTER_F16X3, cb.PROJ_BF16), (cb.BATCH_PERSISTENT, 0, cb.FILTER_FFT32, cb.PROJ_BF16)])
+def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir, mode, group_mb, filt, proj):
     """Batching form: scans of different length, pitch, FOV and matrix size
     give exactly the one-scan-at-a-time results, through the persistent form
     (scans grouped per kernel variant; with a 1 MiB group budget every scan
@@ -575,7 +580,7 @@ def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir, mode, group_m
         ctx.synth_scan_dev(mr, scans.phantom_for(tok, 5), 0, tok["Readings"], raw)
         out = torch.empty(cb.block_sizes(mr).final_stack, dtype=torch.float32, device="cuda")
         return mr, raw, tok["Readings"], out
-    ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
+    ctx.set_options(mu_water=0.02, adaptive_filter=1, filter_mode=filt, proj_dtype=proj)
     for combo in (("both", "both", "both"), ("both", "z", "both")):
         items = [scan(combo[0], 32, 24, 30, 250.0, 0.8, "a.prm"), scan(combo[1], 48, 30, 40, 200.0, 0.6, "b.prm"),
                  scan(combo[2], 16, 20, 12, 300.0, 1.1, "c.prm")]
@@ -593,6 +598,7 @@ def test_reconstruct_batch_matches_single(ctx, tmp_path, mini_dir, mode, group_m
         for (mr, raw, n, out), want in zip(items, singles):
             assert np.array_equal(host(out), want)
             cb.free_metadata(mr)
+    ctx.set_options(filter_mode=cb.FILTER_FFT32, proj_dtype=cb.PROJ_F32)
 
 
 @pytest.mark.parametrize("mode,proj", [(cb.FILTER_BF16X3, cb.PROJ_F32), (cb.FILTER_F16X3, cb.PROJ_F32),
