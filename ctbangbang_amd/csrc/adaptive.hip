@@ -41,20 +41,23 @@ __device__ __forceinline__ float wave_min(float v) {
 }
 
 // View maximum, one workgroup per view (max is order-free, so any reduction
-// order gives the reference's value).  frame % 4 == 0 is checked on the host.
-__global__ __launch_bounds__(256) void af_sup(const float *raw, float *sup, size_t frame) {
+// order gives the reference's value), gridDim.x workgroups looping over the
+// n_proj views.  frame % 4 == 0 is checked on the host.
+__global__ __launch_bounds__(256) void af_sup(const float *raw, float *sup, size_t frame, int n_proj) {
     __shared__ float red[4];
-    const size_t p = blockIdx.x;
-    const float4 *f = reinterpret_cast<const float4 *>(raw + p * frame);
-    float m = 0.0f;   // reference starts from 0.0f (preprocessing.cuh:75)
-    for (size_t i = threadIdx.x; i < frame / 4; i += 256) {
-        const float4 v = f[i];
-        m = fmaxf(fmaxf(m, fmaxf(v.x, v.y)), fmaxf(v.z, v.w));
+    for (int p = blockIdx.x; p < n_proj; p += gridDim.x) {
+        const float4 *f = reinterpret_cast<const float4 *>(raw + static_cast<size_t>(p) * frame);
+        float m = 0.0f;   // reference starts from 0.0f (preprocessing.cuh:75)
+        for (size_t i = threadIdx.x; i < frame / 4; i += 256) {
+            const float4 v = f[i];
+            m = fmaxf(fmaxf(m, fmaxf(v.x, v.y)), fmaxf(v.z, v.w));
+        }
+        m = wave_max(m);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) sup[p] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        __syncthreads();                        // red is rewritten for the next view
     }
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) sup[p] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
 // reference preprocessing.cuh:87-113
@@ -255,13 +258,15 @@ int launch_adaptive(const AdaptiveArgs &a, hipStream_t stream) {
     const int n_avg = odd_window(18.0f / 360.0f, a.n_proj_ffs);
     const int n_range = odd_window(180.0f / 360.0f, a.n_proj_ffs);
     const unsigned g1 = static_cast<unsigned>((a.n_proj + 255) / 256);
+    if (a.max_groups < 0) return CTBB_ERR_SHAPE;
+    const unsigned gsup = static_cast<unsigned>(a.max_groups > 0 && a.max_groups < a.n_proj ? a.max_groups : a.n_proj);
     if (a.fixed_threshold > 0.0f) {
         if (!(a.fixed_threshold < 3.0e38f)) return CTBB_ERR_ARG;
         hipLaunchKernelGGL(af_const, dim3(g1), dim3(256), 0, stream, thr, a.n_proj, a.fixed_threshold);
         // (the view maxima decide which views hot_only writes)
-        if (a.hot_only) hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
+        if (a.hot_only) hipLaunchKernelGGL(af_sup, dim3(gsup), dim3(256), 0, stream, a.raw, sup, frame, a.n_proj);
     } else {
-        hipLaunchKernelGGL(af_sup, dim3(a.n_proj), dim3(256), 0, stream, a.raw, sup, frame);
+        hipLaunchKernelGGL(af_sup, dim3(gsup), dim3(256), 0, stream, a.raw, sup, frame, a.n_proj);
         hipLaunchKernelGGL(af_smooth, dim3(g1), dim3(256), 0, stream, sup, sm, a.n_proj, n_avg);
         // the counting window (at most n_range values) in LDS when it takes <= 16 KB
         const size_t win_bytes = static_cast<size_t>(n_range) * sizeof(float);

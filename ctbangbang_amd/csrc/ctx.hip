@@ -78,6 +78,9 @@ int scratch_address(int device, ctbb::BpView **out) {
 
 struct ctbb_ctx {
     int device = 0;
+    // workgroups of a preparation kernel launched beside a per-block
+    // backprojection: 2 per CU (set at creation)
+    int prep_groups = 0;
     hipStream_t stream = nullptr;
     ctbb_options opt{};
     ctbb::BpView *noalloc_vtab = nullptr;   // this device's `scratch` array
@@ -345,7 +348,8 @@ int ensure_maps(ctbb_ctx *ctx, const recon_metadata *mr) {
 // hot (with d_af): frames whose flag is set come from d_af, the adaptive
 // filter's hot_only output, the others from d_raw.
 int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, int n_views_max = -1,
-             hipStream_t on = nullptr, const unsigned char *hot = nullptr, const float *d_af = nullptr) {
+             hipStream_t on = nullptr, const unsigned char *hot = nullptr, const float *d_af = nullptr,
+             int max_groups = 0) {
     int st = ensure_maps(ctx, mr);
     if (st) return st;
     const ct_geom &cg = mr->cg;
@@ -369,6 +373,7 @@ int do_rebin(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float 
     a.nonfinite = ctx->nonfinite.as<unsigned>();
     a.hot = hot;
     a.raw_af = d_af;
+    a.max_groups = max_groups;
     hipStream_t s = on ? on : ctx->stream;
     EventPair e = ctx->begin("rebin", s);
     st = launch_rebin(a, s);
@@ -548,7 +553,8 @@ int do_finalize(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_image, f
 // hot: write d_out only for the views the filter changes and return their
 // flags in *hot (inside the work buffer, valid until its next use).
 int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, float *d_out, bool vec4 = true,
-                hipStream_t on = nullptr, DevBuf *work = nullptr, const unsigned char **hot = nullptr) {
+                hipStream_t on = nullptr, DevBuf *work = nullptr, const unsigned char **hot = nullptr,
+                int max_groups = 0) {
     const ct_geom &cg = mr->cg;
     AdaptiveArgs a{};
     a.raw = d_raw;
@@ -561,6 +567,7 @@ int do_adaptive(ctbb_ctx *ctx, const recon_metadata *mr, const float *d_raw, flo
     a.vec4 = vec4 ? 1 : 0;
     a.fixed_threshold = ctx->opt.adaptive_threshold;
     a.hot_only = hot ? 1 : 0;
+    a.max_groups = max_groups;
     DevBuf &wb = work ? *work : ctx->af_work;
     int st = wb.ensure(6 * static_cast<size_t>(a.n_proj) * sizeof(float));
     if (st) return st;
@@ -621,11 +628,20 @@ int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out, h
     DevBuf &rb = head ? c->head_rebin : c->rebin_tmp;
     // The adaptive filter writes only the views it changes (C1: about 6%);
     // the rebin reads those from its output and the rest from the pull.
+    // The first kernel of the chain (the view maxima, or the rebin without
+    // the adaptive filter) is released together with a backprojection
+    // launch: with its full grid it took the CUs before every
+    // backprojection workgroup was placed, and those finished late (C1: 0.3
+    // ms per volume with the filter, 3 ms without it).  That kernel runs at
+    // most prep_groups workgroups (2 per CU, looping over its work).  The
+    // rebin after the filter starts once the backprojection is placed and
+    // keeps its full grid (looping workgroups are slower there).
     const unsigned char *hot = nullptr;
     if (c->opt.adaptive_filter) {
         st = af.ensure(n * frame * sizeof(float));
         if (st) return st;
-        st = do_adaptive(c, mr, win, af.as<float>(), false, head, head ? &c->head_work : nullptr, &hot);
+        st = do_adaptive(c, mr, win, af.as<float>(), false, head, head ? &c->head_work : nullptr, &hot,
+                         c->prep_groups);
         if (st) return st;
     }
     st = rb.ensure(sz.rebin * sizeof(float));
@@ -645,7 +661,8 @@ int prep_block(ctbb_ctx *c, recon_metadata *mr, const float *win, void *d_out, h
         views += 1;
         rows += 1;
     }
-    st = do_rebin(c, mr, win, rb.as<float>(), static_cast<int>(views), head, hot, hot ? af.as<float>() : nullptr);
+    st = do_rebin(c, mr, win, rb.as<float>(), static_cast<int>(views), head, hot, hot ? af.as<float>() : nullptr,
+                  c->opt.adaptive_filter ? 0 : c->prep_groups);
     if (st) return st;
     if (head && (hipEventRecord(c->ev_head, head) != hipSuccess ||
                  hipStreamWaitEvent(c->stream, c->ev_head, 0) != hipSuccess))
@@ -739,6 +756,9 @@ int ctbb_ctx_create(int device, ctbb_ctx **out) {
     }
     ctbb_ctx *c = new ctbb_ctx();
     (void)hipGetDevice(&c->device);
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && n_cu > 0)
+        c->prep_groups = 2 * n_cu;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return CTBB_ERR_HIP;

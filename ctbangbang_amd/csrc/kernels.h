@@ -73,6 +73,9 @@ struct RebinArgs {
     // for the frames it changes)
     const unsigned char *hot;
     const float *raw_af;
+    // > 0: at most this many workgroups, each looping over the work (the
+    // rebin then cannot take the CUs a kernel launched beside it needs)
+    int max_groups;
 };
 int launch_rebin(const RebinArgs &a, hipStream_t stream);
 
@@ -106,6 +109,9 @@ struct AdaptiveArgs {
     // exceeds their threshold), flagged in adaptive_hot_flags(work, n_proj);
     // a reader takes the other views from raw (RebinArgs::hot)
     int hot_only;
+    // > 0: the view-maximum kernel runs at most this many workgroups, each
+    // looping over views (see RebinArgs::max_groups)
+    int max_groups;
 };
 // the per-view flags of hot_only inside AdaptiveArgs::work (n_proj bytes)
 inline unsigned char *adaptive_hot_flags(float *work, int n_proj) {

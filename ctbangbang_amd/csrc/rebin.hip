@@ -59,10 +59,25 @@ __device__ __forceinline__ const float *frame_src(const RebinArgs &a, size_t f) 
     return a.hot[f] ? a.raw_af : a.raw;
 }
 
+// The (x, y, z) workgroups of the rebin's grid (x: 256 output channels, y:
+// row, z: kViewsPerThread views), in dispatch order, taken in a stride of
+// gridDim.x by the launched one-dimensional grid (fewer workgroups than the
+// grid has: the rebin then leaves room on every CU for a kernel launched
+// beside it).
+template <class F>
+__device__ __forceinline__ void for_each_block(const RebinArgs &a, F f) {
+    const unsigned gx = static_cast<unsigned>(a.nos + 255) / 256u, gxy = gx * static_cast<unsigned>(a.n_rows);
+    const unsigned total = gxy * static_cast<unsigned>((a.n_proj_out + kViewsPerThread - 1) / kViewsPerThread);
+    for (unsigned vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const unsigned bz = vb / gxy, rem = vb - bz * gxy, br = rem / gx;
+        f(rem - br * gx, br, bz);
+    }
+}
+
 template <bool kHot>
-__global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
+__device__ __forceinline__ void rebin_block(const RebinArgs &a, unsigned bx, unsigned br, unsigned bz, bool &bad) {
+    const int c = static_cast<int>(bx) * 256 + static_cast<int>(threadIdx.x);
+    const int r = static_cast<int>(br);
     if (c >= a.nos) return;
     const size_t frame = static_cast<size_t>(a.n_rows_raw) * a.nch;
     const float *__restrict__ aoff = a.aoff;
@@ -92,9 +107,8 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
         ao1 = aoff[set * a.nos + m1];
     }
 
-    bool bad = false;
     for (int vv = 0; vv < kViewsPerThread; vv++) {
-        const int v = blockIdx.z * kViewsPerThread + vv;
+        const int v = static_cast<int>(bz) * kViewsPerThread + vv;
         if (v >= a.n_proj_out) break;
         const float proj = static_cast<float>(v + a.add);
         float out;
@@ -132,6 +146,12 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
         outp[(static_cast<size_t>(v) * a.n_rows + r) * a.nos + c] = out;
         bad |= !__builtin_isfinite(out);
     }
+}
+
+template <bool kHot>
+__global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
+    bool bad = false;
+    rebin_block<kHot>(a, blockIdx.x, blockIdx.y, blockIdx.z, bad);
     report_nonfinite(a.nonfinite, bad);
 }
 
@@ -139,8 +159,9 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a) {
 // output bases, one template per channel-tap form (kinds 0/1: bilinear in
 // raw channel and azimuth; 2/3: two oversampled-channel taps, each bilinear
 // in azimuth), for pulls and outputs below 4 GiB.  Few registers (<= 32
-// VGPRs), so its waves fit beside a running backprojection (120 VGPRs,
-// 4 waves/SIMD) and the rebin of the next block overlaps it.
+// VGPRs), so its waves fit beside a running backprojection (4 waves per SIMD
+// of 112 VGPRs leave 64 per lane) and the rebin of the next block overlaps
+// it.
 __device__ __forceinline__ float ld(const float *base, unsigned boff) {
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff);
 }
@@ -154,9 +175,9 @@ __device__ __forceinline__ float ld_frame(const RebinArgs &a, unsigned f, unsign
 }
 
 template <bool kSplit, bool kHot>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void rebin32_kernel(RebinArgs a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
+__device__ __forceinline__ void rebin32_block(const RebinArgs &a, unsigned bx, unsigned br, unsigned bz, bool &bad) {
+    const int c = bx * blockDim.x + threadIdx.x;
+    const int r = br;
     if (c >= a.nos) return;
     const unsigned frame_b = static_cast<unsigned>(a.n_rows_raw * a.nch) * 4u;
     int set, j;
@@ -179,10 +200,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void r
         ao1 = a.aoff[set * a.nos + c1];
     }
     const unsigned row_b = static_cast<unsigned>(j * a.nch) * 4u;
-    const int v0 = blockIdx.z * kViewsPerThread;
+    const int v0 = bz * kViewsPerThread;
     unsigned ob = (static_cast<unsigned>(v0 * a.n_rows + r) * static_cast<unsigned>(a.nos) + c) * 4u;
     const unsigned ostep = static_cast<unsigned>(a.n_rows * a.nos) * 4u;
-    bool bad = false;
     for (int vv = 0; vv < kViewsPerThread; vv++) {
         const int v = v0 + vv;
         if (v >= a.n_proj_out) break;
@@ -221,6 +241,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void r
         bad |= !__builtin_isfinite(out);
         ob += ostep;
     }
+}
+
+// One workgroup per (x, y, z) block of the grid.  At most 32 VGPRs (the
+// attribute counts pairs here): two waves per SIMD fit in the 64 per lane a
+// running backprojection leaves.
+template <bool kSplit, bool kHot>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8), amdgpu_num_vgpr(16))) void rebin32_kernel(
+    RebinArgs a) {
+    bool bad = false;
+    rebin32_block<kSplit, kHot>(a, blockIdx.x, blockIdx.y, blockIdx.z, bad);
+    report_nonfinite(a.nonfinite, bad);
+}
+
+// A one-dimensional grid of RebinArgs::max_groups workgroups looping over the
+// blocks, at most 32 VGPRs as above.
+template <bool kSplit, bool kHot>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8), amdgpu_num_vgpr(16))) void rebin32_loop_kernel(
+    RebinArgs a) {
+    bool bad = false;
+    for_each_block(a, [&](unsigned bx, unsigned br, unsigned bz) { rebin32_block<kSplit, kHot>(a, bx, br, bz, bad); });
     report_nonfinite(a.nonfinite, bad);
 }
 
@@ -233,22 +273,36 @@ int launch_rebin(const RebinArgs &a, hipStream_t stream) {
     if (a.add < 0 || a.add + a.n_proj_out > a.n_proj) return CTBB_ERR_SHAPE;
     const int rows_needed = (a.kind == 1 || a.kind == 3) ? 2 * a.n_rows_raw : a.n_rows_raw;
     if (a.n_rows != rows_needed) return CTBB_ERR_SHAPE;
-    dim3 grid((a.nos + 255) / 256, a.n_rows, (a.n_proj_out + kViewsPerThread - 1) / kViewsPerThread);
-    if (grid.z > 65535u || grid.y > 65535u) return CTBB_ERR_SHAPE;
+    const dim3 grid((a.nos + 255) / 256, a.n_rows, (a.n_proj_out + kViewsPerThread - 1) / kViewsPerThread);
+    if (grid.z > 65535u || grid.y > 65535u || a.max_groups < 0) return CTBB_ERR_SHAPE;
+    const unsigned long long blocks = static_cast<unsigned long long>(grid.x) * grid.y * grid.z;
     const unsigned long long raw_bytes = static_cast<unsigned long long>(a.n_proj) * a.n_ffs * a.n_rows_raw * a.nch * 4ull;
     const unsigned long long out_bytes = static_cast<unsigned long long>(a.n_proj_out) * a.n_rows * a.nos * 4ull;
+    const bool hot = a.hot != nullptr;
     if (raw_bytes < (1ull << 32) && out_bytes < (1ull << 32)) {
-        const bool hot = a.hot != nullptr;
-        if (a.kind <= 1 && !hot)
+        if (a.max_groups > 0 && static_cast<unsigned long long>(a.max_groups) < blocks) {
+            const dim3 g1(static_cast<unsigned>(a.max_groups));
+            if (a.kind <= 1 && !hot)
+                hipLaunchKernelGGL((rebin32_loop_kernel<false, false>), g1, dim3(256), 0, stream, a);
+            else if (a.kind <= 1)
+                hipLaunchKernelGGL((rebin32_loop_kernel<false, true>), g1, dim3(256), 0, stream, a);
+            else if (!hot)
+                hipLaunchKernelGGL((rebin32_loop_kernel<true, false>), g1, dim3(256), 0, stream, a);
+            else
+                hipLaunchKernelGGL((rebin32_loop_kernel<true, true>), g1, dim3(256), 0, stream, a);
+        } else if (a.kind <= 1 && !hot) {
             hipLaunchKernelGGL((rebin32_kernel<false, false>), grid, dim3(256), 0, stream, a);
-        else if (a.kind <= 1)
+        } else if (a.kind <= 1) {
             hipLaunchKernelGGL((rebin32_kernel<false, true>), grid, dim3(256), 0, stream, a);
-        else if (!hot)
+        } else if (!hot) {
             hipLaunchKernelGGL((rebin32_kernel<true, false>), grid, dim3(256), 0, stream, a);
-        else
+        } else {
             hipLaunchKernelGGL((rebin32_kernel<true, true>), grid, dim3(256), 0, stream, a);
+        }
     } else {
-        if (a.hot)
+        // (more than 32 VGPRs: long-lived workgroups of it would keep a
+        // backprojection workgroup off their CU, so its grid stays whole)
+        if (hot)
             hipLaunchKernelGGL(rebin_kernel<true>, grid, dim3(256), 0, stream, a);
         else
             hipLaunchKernelGGL(rebin_kernel<false>, grid, dim3(256), 0, stream, a);
