@@ -1586,8 +1586,9 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
                 // FP32 projections, FFT filter: first the adaptive filter and
                 // the rebinning of every block, straight into its slot
                 // (low-register kernels that run beside the backprojection
-                // before), then the filter of every slot in place (the FFT
-                // kernel needs the registers the backprojection holds)
+                // before; the first kernel of each block capped as in
+                // prep_block), then the filter of every slot in place (the
+                // FFT kernel needs the registers the backprojection holds)
                 for (int j = rg.first; j < rg.second && !st; j++) {
                     recon_metadata *m = mrs[jobs[j].s];
                     ctbb::plan_block(m, jobs[j].b);
@@ -1595,11 +1596,13 @@ int batch_persistent(ctbb_ctx *c, int n_scans, recon_metadata *const *mrs, const
                     st = block_window(c, m, d_raws[jobs[j].s], n_raw[jobs[j].s], &win);
                     const unsigned char *hot = nullptr;
                     if (!st && c->opt.adaptive_filter)
-                        st = do_adaptive(c, m, win, c->raw_af.as<float>(), false, nullptr, nullptr, &hot);
+                        st = do_adaptive(c, m, win, c->raw_af.as<float>(), false, nullptr, nullptr, &hot,
+                                         c->prep_groups);
                     if (!st)
                         st = do_rebin(c, m, win, reinterpret_cast<float *>(slot_ptr(jobs[j])),
                                       static_cast<int>(prep_views(m)), nullptr, hot,
-                                      hot ? c->raw_af.as<float>() : nullptr);
+                                      hot ? c->raw_af.as<float>() : nullptr,
+                                      c->opt.adaptive_filter ? 0 : c->prep_groups);
                 }
                 for (int j = rg.first; j < rg.second && !st; j++) {
                     recon_metadata *m = mrs[jobs[j].s];
