@@ -810,6 +810,33 @@ int launch_backproject(const BpArgs &a0, hipStream_t stream, bool beside) {
     });
 }
 
+template <int NZ, int CPT, int TPR, int ZS>
+int resident_c(const BpArgs &a, long *groups) {
+    const size_t lds_bytes = lds_bytes_of<NZ, CPT, TPR, ZS>(a);
+    auto kf = a.rebin_bf16 ? reinterpret_cast<const void *>(bp_kernel_of<NZ, CPT, TPR, ZS, true>())
+                           : reinterpret_cast<const void *>(bp_kernel_of<NZ, CPT, TPR, ZS, false>());
+    const int st = allow_lds(kf, lds_bytes);
+    if (st) return st;
+    int dev = 0, n_cu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 256 * ZS, lds_bytes) != hipSuccess)
+        return CTBB_ERR_HIP;
+    *groups = std::max(1, per_cu) * static_cast<long>(std::max(1, n_cu)) / (32 / (NZ * ZS));
+    return CTBB_OK;
+}
+
+int backproject_resident(const BpArgs &a0, long *groups) {
+    if (!groups) return CTBB_ERR_ARG;
+    BpArgs a = a0;
+    const int id = backproject_variant(a);
+    if (id < 0) return -id;
+    return with_variant(id, [&](auto nz, auto zs, auto cpt, auto tpr) {
+        return resident_c<decltype(nz)::value, decltype(cpt)::value, decltype(tpr)::value, decltype(zs)::value>(
+            a, groups);
+    });
+}
+
 int launch_backproject_persistent(const BpArgs &sample, int variant, const BpQueues &Q, int total_items,
                                   int max_groups, hipStream_t stream) {
     if (!Q.jobs || !Q.band_pre || !Q.counters || Q.n_jobs <= 0 || Q.job_first < 0 || Q.job_end <= Q.job_first ||

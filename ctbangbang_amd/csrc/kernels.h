@@ -35,6 +35,9 @@ struct BpArgs {
 // (then the grid is only the resident workgroups, which stride over the
 // tiles; otherwise one workgroup per tile).
 int launch_backproject(const BpArgs &a, hipStream_t stream, bool beside = false);
+// Workgroups of launch_backproject(a) resident on the device at once (per
+// slice range): *groups.
+int backproject_resident(const BpArgs &a, long *groups);
 
 // Persistent backprojection over the tiles of many blocks (jobs).  The jobs
 // of one launch share the kernel variant (rows, staging shape, slices per
